@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark of the Gross-Pitaevskii Strang split step (NumModelSplit2) at 256³.
+
+Workload: periodic box [0, 2π)³, 256³ sites, V = 0, coeffΔ = -0.5,
+ξ = 2Δ, β = 1/ξ², Δt = 5e-4; ψ0 = e^{iθ} with θ a seeded Gaussian random
+field on 1 <= |k| <= 8 (modal variance ∝ |k|^-2, rms 1).  A step is one full
+Strang step (x, y, z round trips over Δt/2, the nonlinear phase over Δt,
+x, y, z again) of the whole field, which stays resident in HBM.  FP32 is the
+headline ``value``; FP64 is measured side by side.
+
+Prints one JSON line.  ``python bench.py --gpus N --steps K --warmup W``.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "superfluiddynamics.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def bytes_per_site_update(prec: str, ndim: int = 3, with_v: bool = False) -> int:
+    """Algorithmic bytes per site-update: 2D one-axis round trips, each one
+    read and one write of the field; the nonlinear phase is fused."""
+    sc = 8 if prec == "f32" else 16
+    sr = 4 if prec == "f32" else 8
+    return 2 * ndim * 2 * sc + (sr if with_v else 0)
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration recorded with it); None if no summary was committed."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d.get("traffic_bytes_per_launch")
+
+
+def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, kernel_launches):
+    import numpy as np
+    FT = np.float32 if prec == "f32" else np.float64
+    n = psi0.shape[0]
+    g = sfd.Grid((n, n, n), mp["bounds"], FT=FT)
+    f = sfd.Field(g)
+    f.set(psi0)
+    p = sfd.GrossPitaevskiiParameters(pot=sfd.PotentialZero(f), coeffΔ=mp["coeffD"], β=mp["beta"])
+    model = sfd.NumModelSplit2(f, p, mp["dt"], steps, steps)
+    model.steps(max(warmup, 1))
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    model.steps(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_step = e0.elapsed_time(e1) / steps
+    state = f.to_numpy()
+    nrm = sfd.norm(f)
+    # Average duration of one launch of each sweep kernel, on the stream the
+    # kernels run on (torch's current stream is the context's stream).
+    lib = sfd.load()
+    import ctypes
+    ctx = f.ctx()
+    per_axis = []
+    for ax in range(3):
+        k0 = torch.cuda.Event(enable_timing=True)
+        k1 = torch.cuda.Event(enable_timing=True)
+        rc = lib.sfd_lap_rot_axis(ctx.h, ctypes.c_void_p(f.ϕ.data_ptr()), ax, mp["coeffD"], 0.0, mp["dt"] * 0.5, 0)
+        assert rc == 0, lib.sfd_last_error()
+        torch.cuda.synchronize()
+        k0.record()
+        for _ in range(kernel_launches):
+            lib.sfd_lap_rot_axis(ctx.h, ctypes.c_void_p(f.ϕ.data_ptr()), ax, mp["coeffD"], 0.0, mp["dt"] * 0.5, 0)
+        k1.record()
+        torch.cuda.synchronize()
+        per_axis.append(k0.elapsed_time(k1) / kernel_launches)
+    del model, f
+    torch.cuda.empty_cache()
+    return ms_step, per_axis, state, nrm
+
+
+def cpu_leg(psi0, mp, cpu_steps):
+    """The FP64 CPU oracle (C + OpenMP restatement of NumModelSplit2) on the
+    same 256³ workload: one untimed step, then cpu_steps timed steps
+    (the protocol of benchmarks/scalability_gp.jl:36-51)."""
+    import numpy as np
+    from oracle import oracle
+    psi = np.ascontiguousarray(psi0.copy())
+    kw = dict(coeffD=mp["coeffD"], beta=mp["beta"], dt=mp["dt"], order=2)
+    oracle.split_step(psi, mp["bounds"], nsteps=1, **kw)
+    t0 = time.perf_counter()
+    oracle.split_step(psi, mp["bounds"], nsteps=cpu_steps, **kw)
+    t = (time.perf_counter() - t0) / cpu_steps
+    return t, oracle.num_threads()
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-launches", type=int, default=60)
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the whole {args.n}³ domain is resident on a single MI355X; "
+              f"--gpus {args.gpus} is not supported (the slab-decomposed multi-GPU path is not built).",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    import sfdhip as sfd
+    from sfdhip.workloads import metric_params, random_phase_3d
+
+    n = args.n
+    mp = metric_params(n)
+    t0 = time.perf_counter()
+    psi0 = random_phase_3d(n, seed=0)
+    print(f"[bench] initial state {n}³ generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    res = {}
+    states = {}
+    for prec in ("f32", "f64"):
+        ms, per_axis, st, nrm = gpu_leg(sfd, torch, psi0, mp, prec, args.steps, args.warmup, args.kernel_launches)
+        res[prec] = dict(ms=ms, per_axis=per_axis, norm=nrm)
+        states[prec] = st
+        print(f"[bench] {prec}: {ms:.4f} ms/step, sweeps x/y/z {['%.4f' % v for v in per_axis]} ms",
+              file=sys.stderr, flush=True)
+    err32 = float(np.sqrt(np.sum(np.abs(states["f32"].astype(np.complex128) - states["f64"]) ** 2)
+                          / np.sum(np.abs(states["f64"]) ** 2)))
+    del states
+
+    sites = n ** 3
+    out = {}
+    for prec in ("f32", "f64"):
+        r = res[prec]
+        sups = sites / (r["ms"] * 1e-3)
+        b = bytes_per_site_update(prec)
+        out[prec] = dict(site_updates_per_s=sups, ms_per_step=r["ms"], bytes_per_site_update=b,
+                         step_hbm_frac=sups * b / HBM_PEAK,
+                         sweep_ms=dict(x=r["per_axis"][0], y=r["per_axis"][1], z=r["per_axis"][2]))
+
+    # Dominant kernel: the strided-axis sweep (y and z are 4 of the 6 launches).
+    r32 = res["f32"]
+    dom_ms = 0.5 * (r32["per_axis"][1] + r32["per_axis"][2])
+    launch_bytes = 2 * 8 * sites  # one read + one write of the complex FP32 field
+    achieved = launch_bytes / (dom_ms * 1e-3)
+    traffic = load_traffic("traffic_sweep_strided_f32_256.json")
+
+    cpu = None
+    if not args.no_cpu:
+        t_cpu, cores = cpu_leg(psi0, mp, args.cpu_steps)
+        cpu = {"value": sites / t_cpu, "unit": "site-updates/s", "cores": cores, "kind": "port",
+               "sample": f"FP64 CPU oracle (C+OpenMP restatement of NumModelSplit2), {n}³ metric workload, "
+                         f"1 untimed + {args.cpu_steps} timed steps, {t_cpu:.3f} s/step"}
+
+    line = {
+        "metric": "GP split-step site-updates/s at 256³ (FP32 vs FP64) + % HBM roofline",
+        "value": out["f32"]["site_updates_per_s"],
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": out["f32"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"3D GP Strang split (NumModelSplit2), {n}³ periodic [0,2π)³, V=0, "
+                               f"β=1/ξ² (ξ=2Δ), Δt=5e-4, random-phase ψ0 seed 0",
+                   "grid": [n, n, n], "parallelism": "single GPU, whole domain resident"},
+        "fp64": {"value": out["f64"]["site_updates_per_s"], "ms_per_step": out["f64"]["ms_per_step"],
+                 "hbm_frac": out["f64"]["step_hbm_frac"], "sweep_ms": out["f64"]["sweep_ms"]},
+        "f32_over_f64_speedup": out["f64"]["ms_per_step"] / out["f32"]["ms_per_step"],
+        "f32_step_hbm_frac": out["f32"]["step_hbm_frac"],
+        "f32_sweep_ms": out["f32"]["sweep_ms"],
+        "f32_vs_f64_rel_l2": err32,
+        "f32_vs_f64_steps": args.warmup + args.steps,
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "kernel": "k_sweep strided (y/z round trip), FP32, 256³",
+                     "bytes_per_launch": launch_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

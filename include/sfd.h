@@ -1,0 +1,120 @@
+/*
+ * sfd.h -- C ABI of the MI355X (gfx950) Gross-Pitaevskii split-step library
+ * (libsfdhip.so).  Plain pointers and sizes only; every device pointer is a
+ * HIP device allocation owned by the caller (a Julia HIP array, a torch
+ * tensor, or memory from sfd_alloc).
+ *
+ * Data layout is the reference's pen_x layout: complex values interleaved
+ * (re, im), x fastest, extents (nx, ny[, nz]) (Fields.jl:104,147).  The
+ * potential V is a real array of the same extents and precision.
+ *
+ * Every entry point returns SFD_OK (0) or a negative SFD_ERR_* code; the
+ * message of the last failure on the calling host thread is returned by
+ * sfd_last_error().  Work is enqueued asynchronously on the context's
+ * stream, except where a function returns a host scalar (documented below),
+ * which synchronises that stream.  One context per stream; contexts share no
+ * mutable state.
+ */
+#ifndef SFD_H
+#define SFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFD_OK 0
+#define SFD_ERR_NULL (-1)  /* a required pointer is NULL                   */
+#define SFD_ERR_SHAPE (-2) /* unsupported extents (see sfd_ctx_create)     */
+#define SFD_ERR_ARG (-3)   /* invalid scalar argument or option            */
+#define SFD_ERR_OOM (-4)   /* device allocation failed                     */
+#define SFD_ERR_HIP (-5)   /* a HIP runtime call or kernel launch failed   */
+
+#define SFD_F32 1 /* complex FP32 field, FP32 V, FP64 reductions */
+#define SFD_F64 2 /* complex FP64 field, FP64 V                  */
+
+#define SFD_REAL_TIME 0 /* exp(i ...) flows: NumModelSplit1/2                */
+#define SFD_IMAG_TIME 1 /* exp(...) flows + renormalisation after every step */
+
+#define SFD_FORWARD (-1) /* unnormalised exp(-2 pi i jk/n)                   */
+#define SFD_BACKWARD 1   /* exp(+2 pi i jk/n) scaled by 1/n                  */
+
+typedef struct sfd_ctx sfd_ctx;
+
+/* Create a context for one grid: Grid(size, bounds; FT) (Grids.jl:64-129)
+ * plus the FFT plan state of Plan(f; t=FFTPlan()) (Plans.jl:821-999):
+ * wavenumbers fftfreq(n, 2pi/dx), twiddles, and the workspace of the
+ * reductions.  ndim is 2 or 3; n[d] must be a power of two in [2, 1024];
+ * box_min/box_max are the bounds ((xmin,xmax),...).  stream is a
+ * hipStream_t (NULL = the default stream).  Geometry (coordinates, dx,
+ * wavenumbers) is kept in FP64 for both precisions. */
+int sfd_ctx_create(sfd_ctx** out, int ndim, const int64_t* n, const double* box_min,
+                   const double* box_max, int precision, void* stream);
+int sfd_ctx_destroy(sfd_ctx* ctx);
+int sfd_ctx_set_stream(sfd_ctx* ctx, void* stream);
+
+/* Advance psi in place by nsteps split steps.  Replaces timeStep! of
+ * NumModelSplit2 (order = 2, splitting.jl:149-154) and NumModelSplit1
+ * (order = 1, splitting.jl:78-82), i.e. solveLapRot! (splitting.jl:156-203)
+ * and solveNL! (splitting.jl:205-209), for nsteps consecutive calls.
+ * V may be NULL (PotentialZero).  mode SFD_IMAG_TIME runs the imaginary-time
+ * variant: every exponent loses its factor i and the field is renormalised
+ * to norm 1 (normalize!, Fields.jl:196-200) after each step. */
+int sfd_split_step(sfd_ctx* ctx, void* psi, const void* V, double coeffD, double beta,
+                   double Omega, double dt, int64_t nsteps, int order, int mode);
+
+/* solveLapRot!(n, tau) (splitting.jl:156-203): the linear flow over tau,
+ * one round trip per axis in the order x, y[, z].  mode SFD_IMAG_TIME
+ * replaces exp(i ...) by exp(...) (no renormalisation here). */
+int sfd_lap_rot(sfd_ctx* ctx, void* psi, double coeffD, double Omega, double tau, int mode);
+
+/* One axis of solveLapRot!: mul_d!, multiplier, ldiv_d! for axis 0/1/2
+ * (splitting.jl:188-191, 193-196, 198-201). */
+int sfd_lap_rot_axis(sfd_ctx* ctx, void* psi, int axis, double coeffD, double Omega, double tau,
+                     int mode);
+
+/* solveNL!(n, dt) (splitting.jl:205-209): psi *= exp(-i (V + beta|psi|^2) dt)
+ * pointwise; SFD_IMAG_TIME: psi *= exp(-(V + beta|psi|^2) dt).  V may be NULL. */
+int sfd_solve_nl(sfd_ctx* ctx, void* psi, const void* V, double beta, double dt, int mode);
+
+/* sum |psi|^2 accumulated in FP64 (the sum inside norm, Fields.jl:172-180).
+ * Synchronises the stream. */
+int sfd_norm2(sfd_ctx* ctx, const void* psi, double* out_sum);
+
+/* normalize! (Fields.jl:196-200): psi /= norm(psi), norm = sqrt(sum|psi|^2)
+ * sqrt(prod dx).  Writes the norm before scaling to *out_norm when non-NULL.
+ * Synchronises the stream. */
+int sfd_normalize(sfd_ctx* ctx, void* psi, double* out_norm);
+
+/* energy (GrossPitaevskii.jl:61-136): out[0..4] = EOm, EDelta, Ebeta, E, mu
+ * with mu = -EOm + EDelta + 2 Ebeta.  Kinetic and rotation sums are taken in
+ * spectral space by Parseval (one forward sweep per axis) instead of the
+ * reference's gradient fields.  V may be NULL.  Synchronises the stream. */
+int sfd_energy(sfd_ctx* ctx, const void* psi, const void* V, double coeffD, double beta,
+               double Omega, double* out5);
+
+/* One directional transform (mul_x!/mul_y!/mul_z! with direction
+ * SFD_FORWARD, ldiv_x!/ldiv_y!/ldiv_z! with SFD_BACKWARD; Plans.jl:1132-1250).
+ * axis 0 = x, 1 = y, 2 = z.  Input and output are both in pen_x layout
+ * (frequency k of a line stored where sample k sits); in == out is allowed. */
+int sfd_fft_axis(sfd_ctx* ctx, const void* in, void* out, int axis, int direction);
+
+/* Convenience memory helpers on the context's device and stream. */
+int sfd_alloc(sfd_ctx* ctx, size_t bytes, void** out);
+int sfd_free(sfd_ctx* ctx, void* ptr);
+int sfd_upload(sfd_ctx* ctx, void* dst_device, const void* src_host, size_t bytes);
+int sfd_download(sfd_ctx* ctx, void* dst_host, const void* src_device, size_t bytes);
+int sfd_synchronize(sfd_ctx* ctx);
+
+/* Last error message of the calling host thread ("" when none). */
+const char* sfd_last_error(void);
+/* ABI version (major * 100 + minor). */
+int sfd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFD_H */

@@ -1,0 +1,153 @@
+// sfd_dft.hpp -- complex arithmetic and fully unrolled in-register DFTs for
+// gfx950.  Every DFT here is natural-order in, natural-order out; the
+// compiler turns the index permutations into register renaming.
+//
+// Convention (AbstractFFTs, used by the reference through FFTW, Plans.jl:847):
+// forward  X[k] = sum_j x[j] exp(-2 pi i jk/n)        (SIGN = -1)
+// backward x[j] = sum_k X[k] exp(+2 pi i jk/n)        (SIGN = +1, unscaled)
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sfd {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cx {
+    R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename R>
+__device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename R>
+__device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b)
+{
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// a * conj(b)
+template <typename R>
+__device__ __forceinline__ cx<R> cmulc(cx<R> a, cx<R> b)
+{
+    return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+template <typename R>
+__device__ __forceinline__ cx<R> cscale(cx<R> a, R s) { return {a.re * s, a.im * s}; }
+
+// multiply by S*i, S = +1 or -1
+template <int S, typename R>
+__device__ __forceinline__ cx<R> muli(cx<R> v)
+{
+    if constexpr (S > 0) return {-v.im, v.re};
+    else return {v.im, -v.re};
+}
+
+// cos(2 pi m / 64) for m = 0..16, rounded from 36-digit values.
+__device__ constexpr double kQuarterCos[17] = {
+    1.0,
+    0.995184726672196886244836953109479922,
+    0.980785280403230449126182236134239037,
+    0.956940335732208864935797886980269969,
+    0.923879532511286756128183189396788287,
+    0.881921264348355029712756863660388350,
+    0.831469612302545237078788377617905757,
+    0.773010453362736960810906609758469801,
+    0.707106781186547524400844362104849039,
+    0.634393284163645498215171613225493371,
+    0.555570233019602224742830813948532874,
+    0.471396736825997648556387625905254378,
+    0.382683432365089771728459984030398867,
+    0.290284677254462367636192375817395275,
+    0.195090322016128267848284868477022241,
+    0.0980171403295606019941955638886418459,
+    0.0,
+};
+
+template <typename R>
+__device__ __forceinline__ constexpr R cos64(int m)
+{
+    m &= 63;
+    if (m <= 16) return R(kQuarterCos[m]);
+    if (m <= 32) return -R(kQuarterCos[32 - m]);
+    if (m <= 48) return -R(kQuarterCos[m - 32]);
+    return R(kQuarterCos[64 - m]);
+}
+template <typename R>
+__device__ __forceinline__ constexpr R sin64(int m) { return cos64<R>(m - 16); }
+
+// v * exp(SIGN 2 pi i e / N) for a compile-time-foldable e; N divides 64.
+template <int N, int SIGN, typename R>
+__device__ __forceinline__ cx<R> twid(cx<R> v, int e)
+{
+    static_assert(64 % N == 0, "in-register twiddles need N | 64");
+    e = ((e % N) + N) % N;
+    if (e == 0) return v;
+    if (2 * e == N) return {-v.re, -v.im};
+    if (4 * e == N) return muli<SIGN>(v);
+    if (4 * e == 3 * N) return muli<-SIGN>(v);
+    const int m = e * (64 / N);
+    const R c = cos64<R>(m);
+    const R s = R(SIGN) * sin64<R>(m);
+    if ((8 * e) % N == 0) {
+        // odd multiple of pi/4: |c| = |s| = sqrt(1/2)
+        const R h = R(kQuarterCos[8]);
+        const R sc = c > R(0) ? R(1) : R(-1);
+        const R ss = s > R(0) ? R(1) : R(-1);
+        cx<R> r;
+        if (sc == ss) { r.re = h * (v.re - v.im); r.im = h * (v.im + v.re); }
+        else          { r.re = h * (v.re + v.im); r.im = h * (v.im - v.re); }
+        if (sc < R(0)) { r.re = -r.re; r.im = -r.im; }
+        return r;
+    }
+    return {v.re * c - v.im * s, v.re * s + v.im * c};
+}
+
+// In-register DFT of N points (N | 64), natural order in and out.
+template <int N, int SIGN, typename R>
+__device__ __forceinline__ void dft(cx<R>* v)
+{
+    if constexpr (N == 1) {
+        return;
+    } else if constexpr (N == 2) {
+        cx<R> a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    } else if constexpr (N == 4) {
+        cx<R> s02 = v[0] + v[2], d02 = v[0] - v[2];
+        cx<R> s13 = v[1] + v[3], d13 = muli<SIGN>(v[1] - v[3]);
+        v[0] = s02 + s13;
+        v[2] = s02 - s13;
+        v[1] = d02 + d13;
+        v[3] = d02 - d13;
+    } else {
+        // decimation in time, N = P*M: x[P m + p] -> X[k + M q]
+        constexpr int P = (N % 16 == 0) ? 4 : 2;
+        constexpr int M = N / P;
+        cx<R> sub[P][M];
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int m = 0; m < M; ++m) sub[p][m] = v[P * m + p];
+#pragma unroll
+        for (int p = 0; p < P; ++p) dft<M, SIGN, R>(sub[p]);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            cx<R> z[P];
+#pragma unroll
+            for (int p = 0; p < P; ++p) z[p] = twid<N, SIGN, R>(sub[p][k], p * k);
+            dft<P, SIGN, R>(z);
+#pragma unroll
+            for (int q = 0; q < P; ++q) v[k + M * q] = z[q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- reductions
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace sfd

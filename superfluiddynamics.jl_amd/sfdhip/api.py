@@ -1,0 +1,421 @@
+"""Host-side mirror of the reference's Grid / Field / NumModel API for the
+split-step path, backed by libsfdhip.so.
+
+Names, argument meaning and return values follow the reference:
+
+=====================================  =========================================
+reference (file:line)                  here
+=====================================  =========================================
+``Grid(size, bounds; FT, array_type)``  ``Grid(size, bounds, FT=..., array_type=HipArray)``
+  (Grids.jl:64-129)
+``Field(grid, ComplexField())``         ``Field(grid, ComplexField())``  (Fields.jl:94-157)
+``norm(f)``, ``normalize!(f)``          ``norm(f)``, ``normalize_(f)``  (Fields.jl:172-200)
+``GrossPitaevskiiParameters(...)``      same keywords (Parameters.jl:49-55)
+``PotentialZero``, ``PotentialQuadratic``  same (Potentials.jl:17-25, quadratic.jl:48-67)
+``NumModelSplit1/2(f, p, Δt, n, fb)``  same positional arguments (splitting.jl:58-69,129-140)
+``timeStep!(n)`` -> 1 / 2               ``timeStep(n)`` -> 1 / 2  (splitting.jl:78-82,149-154)
+``energy(n, show)``                     ``energy(n, show)`` -> (EΩ, EΔ, Eβ, E) (GrossPitaevskii.jl:61-136)
+``solve!(n; istart)``                   ``solve(n, istart=1)`` (NumModels.jl:23-46, no writers)
+``mul_x!/ldiv_x!`` ... ``_z!``          ``mul_x/ldiv_x`` ... (Plans.jl:1132-1250)
+=====================================  =========================================
+
+Storage: a field is a torch tensor on the current HIP device, complex64
+(FT = float32) or complex128 (FT = float64), shape ``(nz, ny, nx)`` /
+``(ny, nx)`` in C order -- the reference's column-major ``(nx, ny[, nz])``
+pen_x layout, x fastest.  Geometry stays in FP64 whatever FT is.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the image
+    torch = None
+
+
+class HipArray:
+    """The ``array_type`` plug of Grid: fields live in HIP device memory."""
+
+
+class ComplexField:
+    pass
+
+
+class RealField:
+    pass
+
+
+def _require_gpu():
+    if torch is None or not torch.cuda.is_available():
+        raise L.SfdError(L.SFD_ERR_HIP, "no HIP device is available; the split-step path runs on the GPU only")
+
+
+def linrange_coords(lo: float, hi: float, n: int) -> np.ndarray:
+    """``LinRange(lo, hi, n+1)[1:n]`` with Julia's ``(1-t) lo + t hi``, t = i/n
+    (Grids.jl:78-82)."""
+    t = np.arange(n, dtype=np.float64) / float(n)
+    return (1.0 - t) * float(lo) + t * float(hi)
+
+
+def fftfreq(n: int, d: float) -> np.ndarray:
+    """``fftfreq(n, 2π/Δ)`` (Plans.jl:829-830): k * ((2π/Δ)/n)."""
+    mult = (6.283185307179586 / d) / n
+    k = np.arange(n)
+    k = np.where(k < (n + 1) // 2, k, k - n)
+    return k.astype(np.float64) * mult
+
+
+class Grid:
+    """Grid{N,FT,A} (Grids.jl:24-129).  ``size`` and ``bounds`` are x-first."""
+
+    def __init__(self, size, bounds, FT=np.float64, array_type=HipArray):
+        size = tuple(int(s) for s in size)
+        if len(size) not in (2, 3) or len(bounds) != len(size):
+            raise L.ArgError(L.SFD_ERR_ARG, "Grid: 2D or 3D size with one (min, max) per axis")
+        for s in size:
+            if not s > 0:
+                raise L.ArgError(L.SFD_ERR_ARG, "Grid: sizes must be positive (Grids.jl:69)")
+        self.n = size
+        self.min = tuple(float(b[0]) for b in bounds)
+        self.max = tuple(float(b[1]) for b in bounds)
+        self.L = tuple(hi - lo for lo, hi in zip(self.min, self.max))
+        for l in self.L:
+            if not l > 0:
+                raise L.ArgError(L.SFD_ERR_ARG, "Grid: lengths must be positive (Grids.jl:75)")
+        self.Δ = tuple(l / s for l, s in zip(self.L, size))
+        self.FT = np.dtype(FT).type
+        if self.FT not in (np.float32, np.float64):
+            raise L.ArgError(L.SFD_ERR_ARG, "Grid: FT must be Float32 or Float64")
+        if array_type is not HipArray:
+            raise L.ArgError(L.SFD_ERR_ARG, "Grid: this backend provides array_type=HipArray only")
+        self.array_type = array_type
+        self.data = tuple(linrange_coords(lo, hi, s) for lo, hi, s in zip(self.min, self.max, size))
+
+    ndims = property(lambda self: len(self.n))
+    nx = property(lambda self: self.n[0])
+    ny = property(lambda self: self.n[1])
+    nz = property(lambda self: self.n[2])
+    Lx = property(lambda self: self.L[0])
+    Ly = property(lambda self: self.L[1])
+    Lz = property(lambda self: self.L[2])
+    Δx = property(lambda self: self.Δ[0])
+    Δy = property(lambda self: self.Δ[1])
+    Δz = property(lambda self: self.Δ[2])
+    x = property(lambda self: self.data[0])
+    y = property(lambda self: self.data[1])
+    z = property(lambda self: self.data[2])
+
+    @property
+    def shape(self):
+        """numpy/torch shape of a field: reversed extents (x fastest)."""
+        return tuple(reversed(self.n))
+
+    def __repr__(self):
+        return (f"Grid{self.ndims}D\n  ├──────  resolution: {'×'.join(map(str, self.n))}\n"
+                f"  └──────────  domain: " + "×".join(f"[{a},{b}]" for a, b in zip(self.min, self.max)))
+
+
+class _Ctx:
+    """Owns one sfd_ctx (the FFT plan state for a grid and precision)."""
+
+    def __init__(self, grid: Grid):
+        _require_gpu()
+        lib = L.load()
+        self.lib = lib
+        nd = grid.ndims
+        n = (ctypes.c_int64 * 3)(*(list(grid.n) + [1] * (3 - nd)))
+        lo = (ctypes.c_double * 3)(*(list(grid.min) + [0.0] * (3 - nd)))
+        hi = (ctypes.c_double * 3)(*(list(grid.max) + [1.0] * (3 - nd)))
+        prec = L.SFD_F32 if grid.FT is np.float32 else L.SFD_F64
+        h = ctypes.c_void_p()
+        stream = torch.cuda.current_stream().cuda_stream
+        L.check(lib.sfd_ctx_create(ctypes.byref(h), nd, n, lo, hi, prec, ctypes.c_void_p(stream)))
+        self.h = h
+
+    def sync_stream(self):
+        L.check(self.lib.sfd_ctx_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.sfd_ctx_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+
+class Field:
+    """Field{N,1,FT,Complex{FT}} on HIP memory (Fields.jl:94-157)."""
+
+    def __init__(self, g: Grid, t=None, ndims: int = 1):
+        _require_gpu()
+        if t is None:
+            t = ComplexField()
+        if ndims != 1:
+            raise L.ArgError(L.SFD_ERR_ARG, "the split-step path uses scalar fields (ndims=1)")
+        self.g = g
+        complex_ = isinstance(t, ComplexField)
+        if g.FT is np.float32:
+            dt = torch.complex64 if complex_ else torch.float32
+        else:
+            dt = torch.complex128 if complex_ else torch.float64
+        self.data = [torch.zeros(g.shape, dtype=dt, device="cuda")]
+        self._ctx = None
+
+    @property
+    def ϕ(self):
+        return self.data[0]
+
+    @ϕ.setter
+    def ϕ(self, value):
+        self.data[0].copy_(torch.as_tensor(value).reshape(self.g.shape))
+
+    phi = ϕ
+
+    def ctx(self) -> _Ctx:
+        if self._ctx is None:
+            self._ctx = _Ctx(self.g)
+        self._ctx.sync_stream()
+        return self._ctx
+
+    def coords(self):
+        """Broadcast-shaped FP64 numpy coordinates (x, y[, z]) like ``field.x``."""
+        nd = self.g.ndims
+        out = []
+        for a in range(nd):
+            shp = [1] * nd
+            shp[nd - 1 - a] = self.g.n[a]
+            out.append(self.g.data[a].reshape(shp))
+        return tuple(out)
+
+    x = property(lambda self: self.coords()[0])
+    y = property(lambda self: self.coords()[1])
+    z = property(lambda self: self.coords()[2])
+
+    def set(self, values):
+        """Copy a host array (numpy, x-fastest shape) into ϕ."""
+        self.data[0].copy_(torch.as_tensor(np.ascontiguousarray(values)).to(self.data[0].dtype))
+
+    def to_numpy(self):
+        return self.data[0].detach().cpu().numpy()
+
+
+def norm(f: Field) -> float:
+    """norm(f) = sqrt(Σ|ϕ|²) sqrt(ΠΔ) with an FP64 sum (Fields.jl:172-180)."""
+    c = f.ctx()
+    s = ctypes.c_double()
+    L.check(c.lib.sfd_norm2(c.h, ctypes.c_void_p(f.ϕ.data_ptr()), ctypes.byref(s)))
+    return math.sqrt(s.value) * math.sqrt(math.prod(f.g.Δ))
+
+
+def normalize_(f: Field) -> None:
+    """normalize!(f) (Fields.jl:196-200)."""
+    c = f.ctx()
+    L.check(c.lib.sfd_normalize(c.h, ctypes.c_void_p(f.ϕ.data_ptr()), None))
+
+
+normalize = normalize_
+
+
+# ------------------------------------------------------------- parameters
+class PotentialZero:
+    """V = 0 (Potentials.jl:17-25); passed to the kernels as a NULL V."""
+
+    def __init__(self, f: Field):
+        self.f = f
+        self._V = None
+
+    @property
+    def V(self):
+        if self._V is None:
+            self._V = torch.zeros(self.f.g.shape, dtype=_real_dtype(self.f), device="cuda")
+        return self._V
+
+    def device_ptr(self):
+        return None
+
+
+def _real_dtype(f: Field):
+    return torch.float32 if f.g.FT is np.float32 else torch.float64
+
+
+class PotentialQuadratic:
+    """V = ½(1-α)(γx x² + γy y² [+ γz z²]) (quadratic.jl:61-67), evaluated in
+    FP64 with the reference's operation order and rounded once to FT."""
+
+    def __init__(self, f: Field, α=0.0, γx=1.0, γy=1.0, γz=1.0, **kw):
+        α = kw.get("alpha", α)
+        γx, γy, γz = kw.get("gx", γx), kw.get("gy", γy), kw.get("gz", γz)
+        self.f, self.α, self.γx, self.γy, self.γz = f, float(α), float(γx), float(γy), float(γz)
+        co = f.coords()
+        if f.g.ndims == 2:
+            x, y = co
+            Vh = 0.5 * (1 - self.α) * (self.γx * x ** 2 + self.γy * y ** 2)
+        else:
+            x, y, z = co
+            Vh = 0.5 * (1 - self.α) * (self.γx * x ** 2 + self.γy * y ** 2 + self.γz * z ** 2)
+        Vh = np.broadcast_to(Vh, f.g.shape).astype(f.g.FT)
+        self.V = torch.as_tensor(np.ascontiguousarray(Vh)).to("cuda")
+
+    def device_ptr(self):
+        return self.V.data_ptr()
+
+
+class PotentialArray:
+    """Any precomputed real V (same shape and FT as the field)."""
+
+    def __init__(self, f: Field, V):
+        V = torch.as_tensor(np.ascontiguousarray(V) if isinstance(V, np.ndarray) else V)
+        self.f = f
+        self.V = V.to(device="cuda", dtype=_real_dtype(f)).reshape(f.g.shape).contiguous()
+
+    def device_ptr(self):
+        return self.V.data_ptr()
+
+
+class GrossPitaevskiiParameters:
+    """GrossPitaevskiiParameters(; coeffΔ=-0.5, β=1, pot, Ω=0) (Parameters.jl:49-55)."""
+
+    def __init__(self, *, pot, coeffΔ=-0.5, β=1.0, Ω=0.0, **kw):
+        self.pot = pot
+        self.coeffΔ = float(kw.get("coeffD", coeffΔ))
+        self.β = float(kw.get("beta", β))
+        self.Ω = float(kw.get("Omega", Ω))
+
+
+def _vptr(param):
+    p = param.pot.device_ptr() if param.pot is not None else None
+    return ctypes.c_void_p(p) if p else None
+
+
+# ----------------------------------------------------------------- models
+class _SplitBase:
+    order = 2
+    mode = L.SFD_REAL_TIME
+
+    def __init__(self, f: Field, param: GrossPitaevskiiParameters, Δt, niter, freqbckp):
+        if param.pot is not None and getattr(param.pot, "f", f).g is not f.g:
+            raise L.ArgError(L.SFD_ERR_ARG, "the potential belongs to another grid")
+        self.f, self.param = f, param
+        self.Δt, self.niter, self.freqbckp = float(Δt), int(niter), int(freqbckp)
+
+    def steps(self, k: int) -> int:
+        """k consecutive timeStep! calls in one enqueue; returns Σ of their returns."""
+        c = self.f.ctx()
+        p = self.param
+        L.check(c.lib.sfd_split_step(c.h, ctypes.c_void_p(self.f.ϕ.data_ptr()), _vptr(p), p.coeffΔ, p.β, p.Ω,
+                                     self.Δt, int(k), self.order, self.mode))
+        return self.order * int(k)
+
+
+class NumModelSplit1(_SplitBase):
+    """Lie splitting K(Δt) N(Δt) (splitting.jl:58-82)."""
+    order = 1
+
+    def __repr__(self):
+        return f"Splitting Order 1\n  ├───────  time step: {self.Δt}"
+
+
+class NumModelSplit2(_SplitBase):
+    """Strang splitting K(Δt/2) N(Δt) K(Δt/2) (splitting.jl:129-154)."""
+    order = 2
+
+    def __repr__(self):
+        return f"Splitting Order 2\n  ├───────  time step: {self.Δt}"
+
+
+class NumModelSplitImag(_SplitBase):
+    """Imaginary-time split step (normalised gradient flow): the Split1/Split2
+    sweeps with every exponent real, then normalize! after each step.  Its fixed
+    point is the ground state that NumModelBackwardEuler reaches
+    (backward-euler.jl:72-85), up to the O(Δt^order) splitting error."""
+    mode = L.SFD_IMAG_TIME
+
+    def __init__(self, f, param, Δt, niter, freqbckp, order: int = 2):
+        super().__init__(f, param, Δt, niter, freqbckp)
+        if order not in (1, 2):
+            raise L.ArgError(L.SFD_ERR_ARG, "order must be 1 or 2")
+        self.order = order
+
+
+def timeStep(n: _SplitBase) -> int:
+    """timeStep!(n): one split step in place; returns 1 (Split1) or 2 (Split2)."""
+    return n.steps(1)
+
+
+def energy(n: _SplitBase, showEnergy: bool = False):
+    """energy(n, showEnergy) -> (EΩ, EΔ, Eβ, E) (GrossPitaevskii.jl:61-136).
+    The chemical potential is available as ``energy_all(n)[4]``."""
+    EΩ, EΔ, Eβ, E, μ = energy_all(n)
+    if showEnergy:
+        if n.f.g.ndims == 2:
+            print(f"Chemical potential : {μ}")
+        print(f"Angular Momentum Energy : {EΩ}")
+        print(f"Kinetic + Potential Energy : {EΔ}")
+        print(f"Interaction Energy : {Eβ}")
+        print(f"Total Energy: {E}")
+    return EΩ, EΔ, Eβ, E
+
+
+def energy_all(n: _SplitBase):
+    """(EΩ, EΔ, Eβ, E, μ), μ = -EΩ + EΔ + 2Eβ (GrossPitaevskii.jl:79)."""
+    c = n.f.ctx()
+    p = n.param
+    out = (ctypes.c_double * 5)()
+    L.check(c.lib.sfd_energy(c.h, ctypes.c_void_p(n.f.ϕ.data_ptr()), _vptr(p), p.coeffΔ, p.β, p.Ω, out))
+    return tuple(out)
+
+
+def solve(n: _SplitBase, istart: int = 1, verbose: bool = False):
+    """solve!(n; istart) (NumModels.jl:23-46) without the HDF5/VTK writers:
+    niter+1 iterations of (energy, timeStep!); returns the history of
+    (nstep, total, EΩ, EΔ, Eβ, E)."""
+    res = []
+    total = 0
+    for it in range(istart, istart + n.niter + 1):
+        if verbose:
+            print(f"iteration {it}")
+        E = energy(n, verbose)
+        k = timeStep(n)
+        total += k
+        res.append((k, total) + tuple(E))
+    return res
+
+
+# ------------------------------------------------------ directional FFTs
+def _fft(f: Field, out, src, axis: int, direction: int):
+    c = f.ctx()
+    L.check(c.lib.sfd_fft_axis(c.h, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                               axis, direction))
+
+
+def mul_x(out, f: Field, src): _fft(f, out, src, 0, L.SFD_FORWARD)
+def mul_y(out, f: Field, src): _fft(f, out, src, 1, L.SFD_FORWARD)
+def mul_z(out, f: Field, src): _fft(f, out, src, 2, L.SFD_FORWARD)
+def ldiv_x(out, f: Field, src): _fft(f, out, src, 0, L.SFD_BACKWARD)
+def ldiv_y(out, f: Field, src): _fft(f, out, src, 1, L.SFD_BACKWARD)
+def ldiv_z(out, f: Field, src): _fft(f, out, src, 2, L.SFD_BACKWARD)
+
+
+def grid_x(f: Field):
+    """Coordinates and wavenumbers broadcast in pen_x order (Plans.jl:1282-1320);
+    all directional transforms here keep the pen_x layout."""
+    co = f.coords()
+    nd = f.g.ndims
+    xis = []
+    for a in range(nd):
+        shp = [1] * nd
+        shp[nd - 1 - a] = f.g.n[a]
+        xis.append(fftfreq(f.g.n[a], f.g.Δ[a]).reshape(shp))
+    return co + tuple(xis)
+
+
+grid_y = grid_x
+grid_z = grid_x

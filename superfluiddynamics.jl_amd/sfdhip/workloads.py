@@ -1,0 +1,74 @@
+"""Synthetic initial states of the benchmark and parity workloads.
+
+All states are generated once in FP64 on the host with numpy's
+``Generator(PCG64(seed))`` (the reference's MersenneTwister streams are not
+reproducible outside Julia), in the x-fastest layout: numpy shape
+``(ny, nx)`` or ``(nz, ny, nx)``.
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+
+import numpy as np
+
+
+def coords(n, lo, hi):
+    t = np.arange(n, dtype=np.float64) / float(n)
+    return (1.0 - t) * float(lo) + t * float(hi)
+
+
+def discrete_normalize(psi, d):
+    """normalize! (Fields.jl:196-200): divide by sqrt(Σ|ψ|²)·sqrt(ΠΔ)."""
+    nrm = math.sqrt(float(np.sum(np.abs(psi) ** 2))) * math.sqrt(math.prod(d))
+    return psi / nrm
+
+
+def thomas_fermi_2d(n=128, L=8.0, beta=500.0, seed=0, noise=0.1):
+    """2D Thomas-Fermi state on [-L, L)² in V = ½(x²+y²):
+    ψ0 = sqrt(max(0, (μ - V)/β)) e^{iθ}, μ = sqrt(β/π), θ ~ U[-noise, noise]
+    per site, then a discrete normalize!.  Returns (psi, V, bounds)."""
+    x = coords(n, -L, L)
+    X, Y = x[None, :], x[:, None]
+    V = 0.5 * (X ** 2 + Y ** 2)
+    mu = math.sqrt(beta / math.pi)
+    rho = np.maximum(0.0, (mu - V) / beta)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    theta = rng.uniform(-noise, noise, size=(n, n))
+    psi = np.sqrt(rho) * np.exp(1j * theta)
+    d = (2 * L / n, 2 * L / n)
+    return np.ascontiguousarray(discrete_normalize(psi, d)), np.ascontiguousarray(V), ((-L, L), (-L, L))
+
+
+def random_phase_3d(n=256, kmax=8, seed=0, rms=1.0):
+    """Homogeneous condensate ψ0 = e^{iθ} on [0, 2π)³: θ is a Gaussian random
+    field on the integer modes 1 <= |k| <= kmax with modal variance ∝ |k|^-2
+    (complex normal coefficients, real part of the inverse FFT), scaled to
+    the given rms.  Returns psi (complex128, shape (n, n, n))."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    C = np.zeros((n, n, n), dtype=np.complex128)
+    ks = range(-kmax, kmax + 1)
+    for kz in ks:
+        for ky in ks:
+            for kx in ks:
+                k2 = kx * kx + ky * ky + kz * kz
+                if k2 < 1 or k2 > kmax * kmax:
+                    continue
+                a, b = rng.standard_normal(2)
+                C[kz % n, ky % n, kx % n] = (a + 1j * b) / math.sqrt(2.0 * k2)
+    theta = np.fft.ifftn(C).real
+    theta *= rms / math.sqrt(float(np.mean(theta ** 2)))
+    return np.exp(1j * theta)
+
+
+def metric_params(n=256):
+    """Parameters of the 256³ metric workload: box [0, 2π)³, ξ = 2Δ,
+    β = g = 1/ξ², V = 0, coeffΔ = -0.5, Δt = 5e-4."""
+    L = 2 * math.pi
+    dx = L / n
+    xi = 2 * dx
+    return dict(bounds=((0.0, L),) * 3, beta=1.0 / xi ** 2, coeffD=-0.5, dt=5e-4, Omega=0.0)
+
+
+def sha256(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()

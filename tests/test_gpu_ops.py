@@ -1,0 +1,121 @@
+"""GPU checks of the directional FFTs, reductions and full-size invariants."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import mesh, random_potential, rel_l2, smooth_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _field(sfd, psi, bounds, FT=np.float64):
+    g = sfd.Grid(tuple(reversed(psi.shape)), bounds, FT=FT)
+    f = sfd.Field(g)
+    f.set(psi)
+    return f
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("shape", [(32, 16), (256, 64), (16, 32, 8), (64, 128, 32)])
+def test_fft_axis_matches_numpy(gpu, shape, prec):
+    """mul_d! is the unnormalised forward FFT along d, ldiv_d! the normalised
+    inverse (Plans.jl:1132-1250); both in pen_x layout here."""
+    FT = np.float64 if prec == "f64" else np.float32
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(tuple(reversed(shape))) + 1j * rng.standard_normal(tuple(reversed(shape)))
+    bounds = [(0, 1)] * len(shape)
+    f = _field(gpu, a, bounds, FT)
+    out = torch.empty_like(f.ϕ)
+    tol = 1e-13 if prec == "f64" else 2e-6
+    for ax in range(len(shape)):
+        npax = len(shape) - 1 - ax
+        [gpu.mul_x, gpu.mul_y, gpu.mul_z][ax](out, f, f.ϕ)
+        assert rel_l2(out.cpu().numpy(), np.fft.fft(a, axis=npax)) < tol
+        [gpu.ldiv_x, gpu.ldiv_y, gpu.ldiv_z][ax](out, f, f.ϕ)
+        assert rel_l2(out.cpu().numpy(), np.fft.ifft(a, axis=npax)) < tol
+    # in place round trip
+    g = f.ϕ.clone()
+    gpu.mul_y(g, f, g)
+    gpu.ldiv_y(g, f, g)
+    assert rel_l2(g.cpu().numpy(), a) < tol
+
+
+def test_norm_golden_3d(gpu):
+    """Fields.jl:166-170: on Grid((32,32,32), ((-12,12),)*3) the field
+    (-12-x)(12+x)+(-12-y)(12+y)+(-12-z)(12+z) has norm 72951.55177239206."""
+    bounds = ((-12, 12),) * 3
+    x, y, z = mesh((32, 32, 32), bounds)
+    phi = (-12 - x) * (12 + x) + (-12 - y) * (12 + y) + (-12 - z) * (12 + z) + 0j
+    f = _field(gpu, np.ascontiguousarray(np.broadcast_to(phi, (32, 32, 32))), bounds)
+    assert gpu.norm(f) == pytest.approx(72951.55177239206, rel=1e-14)
+
+
+def test_norm_normalize(gpu):
+    """runtests.jl:718-725: ϕ = 1 on 64² [-12,12)² has norm sqrt(Lx Ly)."""
+    bounds = ((-12, 12), (-12, 12))
+    f = _field(gpu, np.ones((64, 64), dtype=np.complex128), bounds)
+    assert gpu.norm(f) == pytest.approx(24.0, rel=1e-14)
+    gpu.normalize(f)
+    assert gpu.norm(f) == pytest.approx(1.0, rel=1e-14)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("case", [((64, 32), ((-6, 6), (-3, 3)), 0.7),
+                                  ((32, 16, 64), ((-4, 4), (-3, 3), (-5, 5)), 0.5)], ids=["2d", "3d"])
+def test_energy_matches_oracle(gpu, orc, case, prec):
+    shape, bounds, Om = case
+    FT = np.float64 if prec == "f64" else np.float32
+    psi = smooth_state(shape, bounds, seed=11)
+    V = random_potential(shape, bounds, amp=5.0)
+    ref = orc.energy(psi, bounds, V=V, coeffD=-0.5, beta=7.0, Omega=Om)
+    f = _field(gpu, psi, bounds, FT)
+    p = gpu.GrossPitaevskiiParameters(pot=gpu.PotentialArray(f, V), coeffΔ=-0.5, β=7.0, Ω=Om)
+    m = gpu.NumModelSplit2(f, p, 1e-3, 1, 1)
+    got = gpu.energy_all(m)
+    scale = max(abs(v) for v in ref)
+    tol = 1e-12 if prec == "f64" else 2e-6
+    for g_, r_ in zip(got, ref):
+        assert abs(g_ - r_) <= tol * scale, (got, ref)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_full_size_invariants(gpu, prec):
+    """256³ metric workload: norm conserved and Split2(Δt) then Split2(-Δt)
+    returns the initial state (the step is unitary and time-reversible)."""
+    from sfdhip.workloads import metric_params, random_phase_3d
+    FT = np.float64 if prec == "f64" else np.float32
+    mp = metric_params(256)
+    psi0 = random_phase_3d(256, seed=0)
+    f = _field(gpu, psi0, mp["bounds"], FT)
+    pot = gpu.PotentialZero(f)
+    p = gpu.GrossPitaevskiiParameters(pot=pot, coeffΔ=-0.5, β=mp["beta"])
+    n0 = gpu.norm(f)
+    fwd = gpu.NumModelSplit2(f, p, mp["dt"], 10, 10)
+    bwd = gpu.NumModelSplit2(f, p, -mp["dt"], 10, 10)
+    fwd.steps(10)
+    n1 = gpu.norm(f)
+    moved = rel_l2(f.to_numpy(), psi0)
+    bwd.steps(10)
+    back = rel_l2(f.to_numpy(), psi0)
+    print(f"256³ {prec}: norm drift {abs(n1 - n0) / n0:.2e}, moved {moved:.3e}, reversal error {back:.3e}")
+    assert moved > 1e-3
+    tol = 1e-12 if prec == "f64" else 1e-5
+    assert abs(n1 - n0) / n0 < tol
+    assert back < (1e-12 if prec == "f64" else 2e-5)
+
+
+def test_error_paths(gpu):
+    g = gpu.Grid((32, 32), ((-1, 1), (-1, 1)))
+    f = gpu.Field(g)
+    lib = gpu.load()
+    import ctypes
+    c = f.ctx()
+    rc = lib.sfd_split_step(c.h, None, None, -0.5, 1.0, 0.0, 1e-3, 1, 2, 0)
+    assert rc == -1
+    rc = lib.sfd_split_step(c.h, ctypes.c_void_p(f.ϕ.data_ptr()), None, -0.5, 1.0, 0.0, 1e-3, 1, 3, 0)
+    assert rc == -3
+    assert b"order" in lib.sfd_last_error()
+    with pytest.raises(gpu.ShapeError):
+        gpu.Field(gpu.Grid((48, 32), ((-1, 1), (-1, 1)))).ctx()
