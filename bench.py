@@ -47,12 +47,14 @@ def load_traffic(name: str):
     return d.get("traffic_bytes_per_launch")
 
 
-def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, kernel_launches):
+def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, kernel_launches, autotune=True):
     import numpy as np
     FT = np.float32 if prec == "f32" else np.float64
     n = psi0.shape[0]
     g = sfd.Grid((n, n, n), mp["bounds"], FT=FT)
     f = sfd.Field(g)
+    tune = f.autotune() if autotune else None
+    variants = [f.ctx().variant(a)[2] for a in range(3)]
     f.set(psi0)
     p = sfd.GrossPitaevskiiParameters(pot=sfd.PotentialZero(f), coeffΔ=mp["coeffD"], β=mp["beta"])
     model = sfd.NumModelSplit2(f, p, mp["dt"], steps, steps)
@@ -75,20 +77,24 @@ def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, kernel_launches):
     ctx = f.ctx()
     per_axis = []
     for ax in range(3):
-        k0 = torch.cuda.Event(enable_timing=True)
-        k1 = torch.cuda.Event(enable_timing=True)
         rc = lib.sfd_lap_rot_axis(ctx.h, ctypes.c_void_p(f.ϕ.data_ptr()), ax, mp["coeffD"], 0.0, mp["dt"] * 0.5, 0)
         assert rc == 0, lib.sfd_last_error()
-        torch.cuda.synchronize()
-        k0.record()
-        for _ in range(kernel_launches):
-            lib.sfd_lap_rot_axis(ctx.h, ctypes.c_void_p(f.ϕ.data_ptr()), ax, mp["coeffD"], 0.0, mp["dt"] * 0.5, 0)
-        k1.record()
-        torch.cuda.synchronize()
-        per_axis.append(k0.elapsed_time(k1) / kernel_launches)
+        reps = []
+        for _ in range(3):
+            k0 = torch.cuda.Event(enable_timing=True)
+            k1 = torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            k0.record()
+            for _ in range(kernel_launches):
+                lib.sfd_lap_rot_axis(ctx.h, ctypes.c_void_p(f.ϕ.data_ptr()), ax, mp["coeffD"], 0.0,
+                                     mp["dt"] * 0.5, 0)
+            k1.record()
+            torch.cuda.synchronize()
+            reps.append(k0.elapsed_time(k1) / kernel_launches)
+        per_axis.append(sorted(reps)[1])
     del model, f
     torch.cuda.empty_cache()
-    return ms_step, per_axis, state, nrm
+    return ms_step, per_axis, state, nrm, variants, tune
 
 
 def cpu_leg(psi0, mp, cpu_steps):
@@ -114,7 +120,8 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel-launches", type=int, default=60)
+    ap.add_argument("--kernel-launches", type=int, default=40)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the whole {args.n}³ domain is resident on a single MI355X; "
@@ -136,8 +143,11 @@ def main(argv=None):
     res = {}
     states = {}
     for prec in ("f32", "f64"):
-        ms, per_axis, st, nrm = gpu_leg(sfd, torch, psi0, mp, prec, args.steps, args.warmup, args.kernel_launches)
-        res[prec] = dict(ms=ms, per_axis=per_axis, norm=nrm)
+        ms, per_axis, st, nrm, variants, tune = gpu_leg(sfd, torch, psi0, mp, prec, args.steps, args.warmup,
+                                                         args.kernel_launches, not args.no_autotune)
+        res[prec] = dict(ms=ms, per_axis=per_axis, norm=nrm, variants=variants, tune=tune)
+        print(f"[bench] {prec}: variants (n1,n2,ts,g) per axis {variants}; autotune ms {tune}",
+              file=sys.stderr, flush=True)
         states[prec] = st
         print(f"[bench] {prec}: {ms:.4f} ms/step, sweeps x/y/z {['%.4f' % v for v in per_axis]} ms",
               file=sys.stderr, flush=True)
@@ -153,7 +163,8 @@ def main(argv=None):
         b = bytes_per_site_update(prec)
         out[prec] = dict(site_updates_per_s=sups, ms_per_step=r["ms"], bytes_per_site_update=b,
                          step_hbm_frac=sups * b / HBM_PEAK,
-                         sweep_ms=dict(x=r["per_axis"][0], y=r["per_axis"][1], z=r["per_axis"][2]))
+                         sweep_ms=dict(x=r["per_axis"][0], y=r["per_axis"][1], z=r["per_axis"][2]),
+                         variants=r["variants"])
 
     # Dominant kernel: the strided-axis sweep (y and z are 4 of the 6 launches).
     r32 = res["f32"]
@@ -186,10 +197,12 @@ def main(argv=None):
                                f"β=1/ξ² (ξ=2Δ), Δt=5e-4, random-phase ψ0 seed 0",
                    "grid": [n, n, n], "parallelism": "single GPU, whole domain resident"},
         "fp64": {"value": out["f64"]["site_updates_per_s"], "ms_per_step": out["f64"]["ms_per_step"],
-                 "hbm_frac": out["f64"]["step_hbm_frac"], "sweep_ms": out["f64"]["sweep_ms"]},
+                 "hbm_frac": out["f64"]["step_hbm_frac"], "sweep_ms": out["f64"]["sweep_ms"],
+                 "variants": out["f64"]["variants"]},
         "f32_over_f64_speedup": out["f64"]["ms_per_step"] / out["f32"]["ms_per_step"],
         "f32_step_hbm_frac": out["f32"]["step_hbm_frac"],
         "f32_sweep_ms": out["f32"]["sweep_ms"],
+        "f32_variants": out["f32"]["variants"],
         "f32_vs_f64_rel_l2": err32,
         "f32_vs_f64_steps": args.warmup + args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",

@@ -101,6 +101,23 @@ int sfd_energy(sfd_ctx* ctx, const void* psi, const void* V, double coeffD, doub
  * (frequency k of a line stored where sample k sits); in == out is allowed. */
 int sfd_fft_axis(sfd_ctx* ctx, const void* in, void* out, int axis, int direction);
 
+/* Autotuner (no counterpart in the reference, whose FFT plans come from
+ * FFTW/CUFFT planning, Plans.jl:847-848,991-993): for each axis, time every
+ * compiled launch variant of the round-trip sweep (line split N = n1*n2,
+ * workgroup shape) on a scratch field and keep the fastest for this context.
+ * scratch: a device buffer of nsites complex values, or NULL to allocate one
+ * temporarily.  times_ms (optional, 24 floats) receives the per-launch time
+ * of variant v of axis a at [a*8 + v] (-1 where none).  Synchronises. */
+int sfd_autotune(sfd_ctx* ctx, void* scratch, float* times_ms);
+
+/* Current launch variant of an axis: its index, the number of variants
+ * compiled for this line length and precision, and (optional) its shape
+ * {n1, n2, ts, g, pipelined}. */
+int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* shape5);
+/* Force a launch variant (index < count); results are identical up to
+ * floating-point rounding. */
+int sfd_ctx_set_variant(sfd_ctx* ctx, int axis, int index);
+
 /* Convenience memory helpers on the context's device and stream. */
 int sfd_alloc(sfd_ctx* ctx, size_t bytes, void** out);
 int sfd_free(sfd_ctx* ctx, void* ptr);

@@ -15,7 +15,7 @@
 #include <vector>
 
 #include "../../include/sfd.h"
-#include "sfd_sweep.hpp"
+#include "sfd_variants.hpp"
 
 using namespace sfd;
 
@@ -48,35 +48,7 @@ int fail(int code, const char* fmt, ...)
         if (r_ != SFD_OK) return r_; \
     } while (0)
 
-// ------------------------------------------------------------ launch shapes
 constexpr int lg2(int n) { return n <= 1 ? 0 : 1 + lg2(n / 2); }
-constexpr int kLdsBudget = 65536;
-
-template <int N>
-struct Split {
-    // N = N1 * N2 with N1 >= N2, both powers of two, N1 <= 32.
-    static constexpr int N2 = 1 << (lg2(N) / 2);
-    static constexpr int N1 = N / N2;
-};
-
-template <typename R, int N>
-struct Cfg {
-    static constexpr int N1 = Split<N>::N1;
-    static constexpr int N2 = Split<N>::N2;
-    static constexpr int CS = (int)sizeof(cx<R>);
-    static constexpr int gc(int g)
-    {
-        return (g > 1 && (g * N2 > 256 || g * (N2 + 1) * N1 * CS > kLdsBudget)) ? gc(g / 2) : g;
-    }
-    static constexpr int G_contig = gc(256);
-    static constexpr int ts(int s) { return (s > 1 && s * N * CS > kLdsBudget) ? ts(s / 2) : s; }
-    static constexpr int TS = ts(16);
-    static constexpr int gs(int g)
-    {
-        return (g < 64 && TS * g * N2 < 64 && 2 * TS * g * N * CS <= kLdsBudget) ? gs(2 * g) : g;
-    }
-    static constexpr int G_strided = gs(1);
-};
 
 // ----------------------------------------------------------- small kernels
 template <typename R>
@@ -177,7 +149,13 @@ struct sfd_ctx {
     int prec = SFD_F64;
     int device = 0;
     hipStream_t stream = nullptr;
+    // per axis: launch variant of k_sweep and its twiddle table
+    int var[3] = {0, 0, 0};
+    VariantInfo vi[3] = {};
     void* tw[3] = {nullptr, nullptr, nullptr};
+    int tuned = 0;
+    int pipe_occ[3] = {0, 0, 0};  // resident workgroups per CU (persistent variants)
+    int ncu = 1;
     double* xi[3] = {nullptr, nullptr, nullptr};
     double* coord[3] = {nullptr, nullptr, nullptr};
     double* partial = nullptr;
@@ -196,16 +174,23 @@ struct sfd_ctx {
 
 namespace {
 
+// Twiddles in the layout k_sweep reads, w = exp(-2 pi i / n):
+//   [0, n):   twA[s][t] = w^(s t), s < n1, t < n2
+//   [n, 2n):  twB[q][s] = w^(q s), q < n2, s < n1
 template <typename R>
-void fill_twiddles(std::vector<cx<R>>& w, int n)
+void fill_twiddles(std::vector<cx<R>>& w, int n, int n1, int n2)
 {
-    w.resize(n);
+    w.resize(2 * (size_t)n);
     const long double pi = 3.14159265358979323846264338327950288L;
-    for (int m = 0; m < n; ++m) {
-        const long double a = -2.0L * pi * (long double)m / (long double)n;
-        w[m].re = (R)cosl(a);
-        w[m].im = (R)sinl(a);
-    }
+    auto put = [&](size_t i, long long m) {
+        const long double a = -2.0L * pi * (long double)(m % n) / (long double)n;
+        w[i].re = (R)cosl(a);
+        w[i].im = (R)sinl(a);
+    };
+    for (int s = 0; s < n1; ++s)
+        for (int t = 0; t < n2; ++t) put((size_t)s * n2 + t, (long long)t * s);
+    for (int q = 0; q < n2; ++q)
+        for (int s = 0; s < n1; ++s) put((size_t)n + (size_t)q * n1 + s, (long long)s * q);
 }
 
 int upload_sync(void* dst, const void* src, size_t bytes)
@@ -279,100 +264,94 @@ int ensure_partial(sfd_ctx* c, long long need)
     return SFD_OK;
 }
 
-// Launch geometry of one sweep: returns grid/block and fills the line fields
-// of the argument block.
+// Launch geometry of one sweep under the axis's current variant: returns
+// grid/block and fills the line fields of the argument block.
 struct Launch {
     dim3 grid, block;
     long long nblocks;
 };
 
-template <typename R, int N>
-Launch sweep_geometry(const sfd_ctx* c, int axis, SweepArgs& a)
+Launch sweep_geometry(const sfd_ctx* c, int axis, SweepArgs& a, bool round_trip)
 {
-    using C = Cfg<R, N>;
+    const VariantInfo& v = c->vi[axis];
+    const int NL = v.ts * v.g;
     Launch L;
     const long long nx = c->n[0], ny = c->n[1], nz = c->n[2];
     if (axis == 0) {
-        constexpr int NL = C::G_contig;
         a.S = 1;
         a.nlines = ny * nz;
-        L.grid = dim3((unsigned)((a.nlines + NL - 1) / NL), 1, 1);
-        L.block = dim3(NL * C::N2, 1, 1);
+        a.ntiles = (a.nlines + NL - 1) / NL;
+        L.grid = dim3((unsigned)a.ntiles, 1, 1);
     } else {
-        constexpr int NL = C::TS * C::G_strided;
         a.ncols = (int)nx;
         if (axis == 1) { a.S = nx; a.OS = nx * ny; }
         else           { a.S = nx * ny; a.OS = nx; }
         const long long others = axis == 1 ? nz : ny;
-        L.grid = dim3((unsigned)((nx + NL - 1) / NL), (unsigned)others, 1);
-        L.block = dim3(NL * C::N2, 1, 1);
+        a.ntx = (int)((nx + NL - 1) / NL);
+        a.ntiles = (long long)a.ntx * others;
+        L.grid = dim3((unsigned)a.ntx, (unsigned)others, 1);
     }
+    if (v.pipe && round_trip) {
+        // persistent: as many workgroups as stay resident
+        const long long resident = (long long)c->pipe_occ[axis] * c->ncu;
+        L.grid = dim3((unsigned)(a.ntiles < resident ? a.ntiles : resident), 1, 1);
+    }
+    L.block = dim3(NL * v.n2, 1, 1);
     L.nblocks = (long long)L.grid.x * L.grid.y;
     return L;
 }
 
-template <typename R, int N, int MODE>
-int launch_n(sfd_ctx* c, int axis, SweepArgs a)
+template <typename R, int MODE>
+int launch_sweep(sfd_ctx* c, int axis, SweepArgs a)
 {
-    using C = Cfg<R, N>;
-    Launch L = sweep_geometry<R, N>(c, axis, a);
-    if (axis == 0)
-        hipLaunchKernelGGL((k_sweep<R, C::N1, C::N2, 1, C::G_contig, MODE>), L.grid, L.block, 0, c->stream, a);
-    else
-        hipLaunchKernelGGL((k_sweep<R, C::N1, C::N2, C::TS, C::G_strided, MODE>), L.grid, L.block, 0, c->stream,
-                           a);
+    Launch L = sweep_geometry(c, axis, a, MODE == kRoundTrip);
+    a.tw = c->tw[axis];
+    if (!launch_variant<R, MODE>(c->n[axis], axis == 0, c->var[axis], L.grid, L.block, c->stream, a))
+        return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", c->var[axis], c->n[axis]);
     SFD_HIP(hipGetLastError());
     return SFD_OK;
 }
 
-template <typename R, int MODE>
-int launch_sweep(sfd_ctx* c, int axis, const SweepArgs& a)
-{
-    switch (c->n[axis]) {
-        case 2: return launch_n<R, 2, MODE>(c, axis, a);
-        case 4: return launch_n<R, 4, MODE>(c, axis, a);
-        case 8: return launch_n<R, 8, MODE>(c, axis, a);
-        case 16: return launch_n<R, 16, MODE>(c, axis, a);
-        case 32: return launch_n<R, 32, MODE>(c, axis, a);
-        case 64: return launch_n<R, 64, MODE>(c, axis, a);
-        case 128: return launch_n<R, 128, MODE>(c, axis, a);
-        case 256: return launch_n<R, 256, MODE>(c, axis, a);
-        case 512: return launch_n<R, 512, MODE>(c, axis, a);
-        case 1024: return launch_n<R, 1024, MODE>(c, axis, a);
-        default: return fail(SFD_ERR_SHAPE, "unsupported line length %d", c->n[axis]);
-    }
-}
-
-template <typename R, int N, int MODE>
-long long nblocks_n(const sfd_ctx* c, int axis)
+long long sweep_nblocks(const sfd_ctx* c, int axis, bool round_trip = true)
 {
     SweepArgs a{};
-    return sweep_geometry<R, N>(c, axis, a).nblocks;
+    return sweep_geometry(c, axis, a, round_trip).nblocks;
 }
 
+// Select launch variant `idx` for `axis` and (re)build the twiddle table of
+// its line split.
 template <typename R>
-long long sweep_nblocks(const sfd_ctx* c, int axis)
+int set_variant(sfd_ctx* c, int axis, int idx)
 {
-    switch (c->n[axis]) {
-        case 2: return nblocks_n<R, 2, 0>(c, axis);
-        case 4: return nblocks_n<R, 4, 0>(c, axis);
-        case 8: return nblocks_n<R, 8, 0>(c, axis);
-        case 16: return nblocks_n<R, 16, 0>(c, axis);
-        case 32: return nblocks_n<R, 32, 0>(c, axis);
-        case 64: return nblocks_n<R, 64, 0>(c, axis);
-        case 128: return nblocks_n<R, 128, 0>(c, axis);
-        case 256: return nblocks_n<R, 256, 0>(c, axis);
-        case 512: return nblocks_n<R, 512, 0>(c, axis);
-        case 1024: return nblocks_n<R, 1024, 0>(c, axis);
-        default: return 0;
+    const int n = c->n[axis];
+    VariantInfo vi;
+    if (!variant_info<R>(n, axis == 0, idx, &vi))
+        return fail(SFD_ERR_ARG, "axis %d: variant %d does not exist (%d variants)", axis, idx,
+                    n_variants<R>(n, axis == 0));
+    c->pipe_occ[axis] = vi.pipe ? variant_occupancy<R>(n, axis == 0, idx) : 0;
+    if (c->tw[axis] && c->vi[axis].n1 == vi.n1) {
+        c->var[axis] = idx;
+        c->vi[axis] = vi;
+        return SFD_OK;
     }
+    std::vector<cx<R>> w;
+    fill_twiddles<R>(w, n, vi.n1, vi.n2);
+    if (c->tw[axis]) {
+        SFD_HIP(hipStreamSynchronize(c->stream));
+        SFD_HIP(hipFree(c->tw[axis]));
+        c->tw[axis] = nullptr;
+    }
+    SFD_HIP(hipMalloc(&c->tw[axis], w.size() * sizeof(cx<R>)));
+    SFD_TRY(upload_sync(c->tw[axis], w.data(), w.size() * sizeof(cx<R>)));
+    c->var[axis] = idx;
+    c->vi[axis] = vi;
+    return SFD_OK;
 }
 
 template <typename R>
 int base_args(sfd_ctx* c, int axis, SweepArgs& a)
 {
     std::memset(&a, 0, sizeof a);
-    a.tw = c->tw[axis];
     return SFD_OK;
 }
 
@@ -397,7 +376,7 @@ int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, doubl
     a.dt = dt;
     a.scale = c->scal;
     if (flags & kFlagNorm) {
-        SFD_TRY(ensure_partial(c, 5 * sweep_nblocks<R>(c, axis)));
+        SFD_TRY(ensure_partial(c, 5 * sweep_nblocks(c, axis)));
         a.partial = c->partial;
     }
     return launch_sweep<R, kRoundTrip>(c, axis, a);
@@ -439,7 +418,7 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
     const double tau = order == 2 ? dt * 0.5 : dt;
     const int nlflags = kFlagNL | (imag ? kFlagImag : 0);
     // Reduction workspace for the per-step renormalisation.
-    if (imag) SFD_TRY(ensure_partial(c, 5 * sweep_nblocks<R>(c, D - 1)));
+    if (imag) SFD_TRY(ensure_partial(c, 5 * sweep_nblocks(c, D - 1)));
     bool pending = false;
     for (int64_t s = 0; s < nsteps; ++s) {
         for (int h = 0; h < order; ++h) {
@@ -452,7 +431,7 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
             }
         }
         if (imag) {
-            const long long nb = sweep_nblocks<R>(c, D - 1);
+            const long long nb = sweep_nblocks(c, D - 1);
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nb, 1, c->scal + 1,
                                c->dvol(), c->scal);
             SFD_HIP(hipGetLastError());
@@ -481,7 +460,7 @@ int energy_impl(sfd_ctx* c, const void* psi, const void* V, double cD, double be
         a.flags = ax == 0 ? kFlagPointwise : 0;
         if (ax == 0 && c->ndim >= 2) { a.coord = c->coord[1]; a.coord_mod = c->n[1]; }
         if (ax == 1) a.coord = c->coord[0];
-        const long long nb = sweep_nblocks<R>(c, ax);
+        const long long nb = sweep_nblocks(c, ax, false);
         SFD_TRY(ensure_partial(c, 5 * nb));
         a.partial = c->partial;
         SFD_TRY(launch_sweep<R, kEnergy>(c, ax, a));
@@ -520,15 +499,66 @@ int fft_impl(sfd_ctx* c, const void* in, void* out, int axis, int dir)
     return launch_sweep<R, kInverse>(c, axis, a);
 }
 
+// Autotuner: for every axis with more than one compiled variant, time the
+// identity round trip (tau = 0) of each variant on a scratch field and keep
+// the fastest.  Each candidate gets 2 warm-up launches, then the best of 3
+// timed batches of 8 launches (HIP events on the context's stream).
+template <typename R>
+int autotune_impl(sfd_ctx* c, void* scratch, float* times_out)
+{
+    void* buf = scratch;
+    bool own = false;
+    if (!buf) {
+        SFD_HIP(hipMalloc(&buf, (size_t)c->nsite * sizeof(cx<R>)));
+        own = true;
+    }
+    int rc = SFD_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(SFD_ERR_HIP, "hipEventCreate failed");
+    }
+    if (rc == SFD_OK && hipMemsetAsync(buf, 0, (size_t)c->nsite * sizeof(cx<R>), c->stream) != hipSuccess)
+        rc = fail(SFD_ERR_HIP, "hipMemsetAsync failed");
+    for (int ax = 0; rc == SFD_OK && ax < c->ndim; ++ax) {
+        const int nv = n_variants<R>(c->n[ax], ax == 0);
+        int best = 0;
+        float best_ms = 1e30f;
+        for (int v = 0; rc == SFD_OK && v < nv; ++v) {
+            if ((rc = set_variant<R>(c, ax, v)) != SFD_OK) break;
+            float vbest = 1e30f;
+            for (int w = 0; rc == SFD_OK && w < 2; ++w)
+                rc = round_trip<R>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, 0, 0.0, 0.0);
+            for (int rep = 0; rc == SFD_OK && rep < 3; ++rep) {
+                (void)hipEventRecord(e0, c->stream);
+                for (int k = 0; rc == SFD_OK && k < 8; ++k)
+                    rc = round_trip<R>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, 0, 0.0, 0.0);
+                (void)hipEventRecord(e1, c->stream);
+                if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(SFD_ERR_HIP, "event sync failed"); break; }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms / 8 < vbest) vbest = ms / 8;
+            }
+            if (times_out) times_out[ax * 8 + (v < 8 ? v : 7)] = vbest;
+            if (vbest < best_ms) { best_ms = vbest; best = v; }
+        }
+        if (rc == SFD_OK) rc = set_variant<R>(c, ax, best);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (own) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(buf);
+    }
+    if (rc == SFD_OK) c->tuned = 1;
+    return rc;
+}
+
 template <typename R>
 int init_tables(sfd_ctx* c)
 {
     for (int ax = 0; ax < c->ndim; ++ax) {
         const int n = c->n[ax];
-        std::vector<cx<R>> w;
-        fill_twiddles<R>(w, n);
-        SFD_HIP(hipMalloc(&c->tw[ax], n * sizeof(cx<R>)));
-        SFD_TRY(upload_sync(c->tw[ax], w.data(), n * sizeof(cx<R>)));
+        SFD_TRY(set_variant<R>(c, ax, 0));
         std::vector<double> xi(n), co(n);
         for (int i = 0; i < n; ++i) {
             xi[i] = fft_freq(n, c->d[ax], i);
@@ -600,6 +630,9 @@ int sfd_ctx_create(sfd_ctx** out, int ndim, const int64_t* n, const double* box_
         delete c;
         return fail(SFD_ERR_HIP, "hipGetDevice failed: %s", hipGetErrorString(e));
     }
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        c->ncu < 1)
+        c->ncu = 1;
     int rc = precision == SFD_F32 ? init_tables<float>(c) : init_tables<double>(c);
     if (rc != SFD_OK) {
         release(c);
@@ -727,6 +760,40 @@ int sfd_fft_axis(sfd_ctx* ctx, const void* in, void* out, int axis, int directio
     if (direction != SFD_FORWARD && direction != SFD_BACKWARD) return fail(SFD_ERR_ARG, "direction must be -1 or 1");
     if (ctx->prec == SFD_F32) return fft_impl<float>(ctx, in, out, axis, direction);
     return fft_impl<double>(ctx, in, out, axis, direction);
+}
+
+int sfd_autotune(sfd_ctx* ctx, void* scratch, float* times_ms)
+{
+    SFD_TRY(check_ctx(ctx));
+    if (times_ms)
+        for (int i = 0; i < 24; ++i) times_ms[i] = -1.f;
+    if (ctx->prec == SFD_F32) return autotune_impl<float>(ctx, scratch, times_ms);
+    return autotune_impl<double>(ctx, scratch, times_ms);
+}
+
+int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* shape5)
+{
+    if (!ctx || !index || !count) return fail(SFD_ERR_NULL, "sfd_ctx_get_variant: null argument");
+    if (axis < 0 || axis >= ctx->ndim) return fail(SFD_ERR_ARG, "axis %d out of range", axis);
+    *index = ctx->var[axis];
+    *count = ctx->prec == SFD_F32 ? n_variants<float>(ctx->n[axis], axis == 0)
+                                  : n_variants<double>(ctx->n[axis], axis == 0);
+    if (shape5) {
+        shape5[0] = ctx->vi[axis].n1;
+        shape5[1] = ctx->vi[axis].n2;
+        shape5[2] = ctx->vi[axis].ts;
+        shape5[3] = ctx->vi[axis].g;
+        shape5[4] = ctx->vi[axis].pipe;
+    }
+    return SFD_OK;
+}
+
+int sfd_ctx_set_variant(sfd_ctx* ctx, int axis, int index)
+{
+    SFD_TRY(check_ctx(ctx));
+    if (axis < 0 || axis >= ctx->ndim) return fail(SFD_ERR_ARG, "axis %d out of range", axis);
+    if (ctx->prec == SFD_F32) return set_variant<float>(ctx, axis, index);
+    return set_variant<double>(ctx, axis, index);
 }
 
 int sfd_alloc(sfd_ctx* ctx, size_t bytes, void** out)

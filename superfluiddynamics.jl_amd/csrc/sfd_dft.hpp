@@ -141,6 +141,75 @@ __device__ __forceinline__ void dft(cx<R>* v)
     }
 }
 
+// ----------------------------------------------------------- sine/cosine
+// sin and cos of one argument for the nonlinear phase, by Cody-Waite
+// argument reduction (x = k pi/2 + r, pi/2 split in three parts applied with
+// fused multiply-adds; W. J. Cody and W. Waite, "Software Manual for the
+// Elementary Functions", 1980) and Taylor series on |r| <= pi/4, truncated
+// where the next term is below half an ulp.  Arguments beyond the range in
+// which the three-part reduction is exact go to the library routine.
+__device__ __forceinline__ void sincos_cw(float x, float* sp, float* cp)
+{
+    if (!(fabsf(x) <= 1.0e4f)) {
+        sincosf(x, sp, cp);
+        return;
+    }
+    const float k = rintf(x * 0.636619746685028076171875f);
+    float r = fmaf(-k, 1.57079637050628662109375f, x);
+    r = fmaf(-k, -4.37113882867379300296306610107421875e-8f, r);
+    r = fmaf(-k, -1.7151245100058819e-15f, r);
+    const float r2 = r * r;
+    float ps = fmaf(r2, 2.7557319223985893e-6f, -1.984126984126984e-4f);
+    ps = fmaf(r2, ps, 8.333333333333333e-3f);
+    ps = fmaf(r2, ps, -1.6666666666666666e-1f);
+    const float sn = fmaf(r * r2, ps, r);
+    float pc = fmaf(r2, -2.755731922398589e-7f, 2.48015873015873e-5f);
+    pc = fmaf(r2, pc, -1.388888888888889e-3f);
+    pc = fmaf(r2, pc, 4.1666666666666664e-2f);
+    pc = fmaf(r2, pc, -0.5f);
+    const float cs = fmaf(r2, pc, 1.0f);
+    const int q = (int)k & 3;
+    const float a = (q & 1) ? cs : sn;
+    const float b = (q & 1) ? sn : cs;
+    *sp = (q & 2) ? -a : a;
+    *cp = ((q + 1) & 2) ? -b : b;
+}
+
+__device__ __forceinline__ void sincos_cw(double x, double* sp, double* cp)
+{
+    if (!(fabs(x) <= 1.0e6)) {
+        sincos(x, sp, cp);
+        return;
+    }
+    const double k = rint(x * 0.6366197723675814);
+    double r = fma(-k, 1.5707963267948966, x);
+    r = fma(-k, 6.123233995736766e-17, r);
+    r = fma(-k, -1.4973849048591698e-33, r);
+    const double r2 = r * r;
+    double ps = fma(r2, 2.8114572543455206e-15, -7.647163731819816e-13);
+    ps = fma(r2, ps, 1.6059043836821613e-10);
+    ps = fma(r2, ps, -2.505210838544172e-08);
+    ps = fma(r2, ps, 2.7557319223985893e-06);
+    ps = fma(r2, ps, -0.0001984126984126984);
+    ps = fma(r2, ps, 0.008333333333333333);
+    ps = fma(r2, ps, -0.16666666666666666);
+    const double sn = fma(r * r2, ps, r);
+    double pc = fma(r2, -1.5619206968586225e-16, 4.779477332387385e-14);
+    pc = fma(r2, pc, -1.1470745597729725e-11);
+    pc = fma(r2, pc, 2.08767569878681e-09);
+    pc = fma(r2, pc, -2.755731922398589e-07);
+    pc = fma(r2, pc, 2.48015873015873e-05);
+    pc = fma(r2, pc, -0.001388888888888889);
+    pc = fma(r2, pc, 0.041666666666666664);
+    pc = fma(r2, pc, -0.5);
+    const double cs = fma(r2, pc, 1.0);
+    const int q = (int)(long long)k & 3;
+    const double a = (q & 1) ? cs : sn;
+    const double b = (q & 1) ? sn : cs;
+    *sp = (q & 2) ? -a : a;
+    *cp = ((q + 1) & 2) ? -b : b;
+}
+
 // ---------------------------------------------------------------- reductions
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)

@@ -60,6 +60,8 @@ struct SweepArgs {
     int coord_mod;          // contiguous: coord index = line % coord_mod
     double beta;            // NL coefficient
     double dt;              // NL time step
+    long long ntiles;       // workgroup tiles of the launch
+    int ntx;                // strided flavour: x-tiles per "other" index
 };
 
 template <typename R, int N1, int N2, int TS, int G>
@@ -92,8 +94,7 @@ __device__ __forceinline__ void nl_phase(cx<R>& v, R vpot, R beta, R dt, bool im
         v.im *= e;
     } else {
         R s, c;
-        if constexpr (sizeof(R) == 4) sincosf(th, &s, &c);
-        else sincos(th, &s, &c);
+        sincos_cw(th, &s, &c);
         // exp(-i th) * v
         const cx<R> m = {c, -s};
         v = cmul(m, v);
@@ -120,12 +121,191 @@ __device__ __forceinline__ void block_partials(double (&acc)[K], double* partial
     }
 }
 
+// Geometry of one workgroup tile: a uniform base (SGPRs) plus a 32-bit lane
+// offset; element j of the lane's line sits at base + lane + j*jstride.
+// Lanes past the end of the field read a clamped (valid) line and never
+// store, so no load needs a predicate.
+struct TileGeom {
+    long long ubase, jstride, kstride;
+    unsigned lane;
+    int row;
+    long long coord_idx;
+    bool active;
+};
+
+template <int N, int N2, int TS, int NL>
+__device__ __forceinline__ TileGeom tile_geom(const SweepArgs& a, long long tile, int L, int t)
+{
+    TileGeom g;
+    g.row = 0;
+    g.coord_idx = 0;
+    if constexpr (TS == 1) {
+        const long long line0 = tile * NL;
+        const long long rem = a.nlines - line0;
+        const int valid = rem < NL ? (int)rem : NL;
+        const int Lc = L < valid ? L : valid - 1;
+        g.active = L < valid;
+        g.ubase = line0 * N;
+        g.lane = (unsigned)(Lc * N + t);
+        g.jstride = N2;
+        g.kstride = 1;
+        const long long line = line0 + Lc;
+        if (a.row_mod > 0) g.row = (int)(line % a.row_mod);
+        if (a.coord_mod > 0) g.coord_idx = line % a.coord_mod;
+    } else {
+        const long long o = tile / a.ntx;
+        const int col0 = (int)(tile - o * a.ntx) * NL;
+        const int rem = a.ncols - col0;
+        const int valid = rem < NL ? rem : NL;
+        const int Lc = L < valid ? L : valid - 1;
+        g.active = L < valid;
+        g.ubase = (long long)col0 + o * a.OS;
+        g.lane = (unsigned)Lc + (unsigned)t * (unsigned)a.S;
+        g.jstride = (long long)N2 * a.S;
+        g.kstride = a.S;
+        if (a.row_from_col) g.row = col0 + Lc;
+        g.coord_idx = col0 + Lc;
+    }
+    return g;
+}
+
+template <typename T>
+__device__ __forceinline__ T& lane_ref(T* base, long long off, unsigned lane)
+{
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base + off) + (size_t)lane * sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ const T& lane_ref(const T* base, long long off, unsigned lane)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base + off) + (size_t)lane * sizeof(T));
+}
+
+template <typename R, int N1>
+__device__ __forceinline__ void load_line(cx<R> (&v)[N1], const cx<R>* psi, const TileGeom& g)
+{
+    const cx<R>* b = psi + g.ubase;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) v[j] = lane_ref(b, j * g.jstride, g.lane);
+}
+
+template <typename R, int N1>
+__device__ __forceinline__ void load_pot(R (&vp)[N1], const R* V, const TileGeom& g)
+{
+    if (V) {
+        const R* b = V + g.ubase;
+#pragma unroll
+        for (int j = 0; j < N1; ++j) vp[j] = lane_ref(b, j * g.jstride, g.lane);
+    } else {
+#pragma unroll
+        for (int j = 0; j < N1; ++j) vp[j] = R(0);
+    }
+}
+
+template <typename R, int N1>
+__device__ __forceinline__ void store_line(const cx<R> (&v)[N1], cx<R>* psi, const TileGeom& g)
+{
+    if (g.active) {
+        cx<R>* b = psi + g.ubase;
+#pragma unroll
+        for (int j = 0; j < N1; ++j) lane_ref(b, j * g.jstride, g.lane) = v[j];
+    }
+}
+
+// Forward transform of the tile's lines: phase A (N1-point DFTs, twiddles),
+// LDS exchange, phase B (N2-point DFTs).  On return w[r][k2] holds
+// X[s + N1 k2] of row s = t + N2 r.
+template <typename R, int N1, int N2, int TS, int G>
+__device__ __forceinline__ void forward_ab(cx<R> (&v)[N1], cx<R> (&w)[N1 / N2][N2], cx<R>* lds, const cx<R>* twA,
+                                           int c, int g, int t)
+{
+    using Sh = SweepShape<R, N1, N2, TS, G>;
+    dft<N1, -1, R>(v);
+#pragma unroll
+    for (int s = 1; s < N1; ++s) v[s] = cmul(v[s], twA[s * N2]);
+#pragma unroll
+    for (int s = 0; s < N1; ++s) lds[Sh::slot(c, g, s, t)] = v[s];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < N1 / N2; ++r) {
+#pragma unroll
+        for (int q = 0; q < N2; ++q) w[r][q] = lds[Sh::slot(c, g, t + N2 * r, q)];
+        dft<N2, -1, R>(w[r]);
+    }
+}
+
+// Inverse transform (unscaled): inverse phase B, conjugate twiddles, LDS
+// exchange, inverse phase A; on return v[j] = N * x[N2 j + t].  Each lane
+// only rewrites the LDS slots it read in forward_ab, so no barrier is needed
+// before the writes.
+template <typename R, int N1, int N2, int TS, int G>
+__device__ __forceinline__ void inverse_ba(cx<R> (&w)[N1 / N2][N2], cx<R> (&v)[N1], cx<R>* lds, const cx<R>* twB,
+                                           int c, int g, int t)
+{
+    using Sh = SweepShape<R, N1, N2, TS, G>;
+#pragma unroll
+    for (int r = 0; r < N1 / N2; ++r) {
+        dft<N2, +1, R>(w[r]);
+        const int s = t + N2 * r;
+#pragma unroll
+        for (int q = 0; q < N2; ++q) {
+            const cx<R> z = (q == 0) ? w[r][q] : cmulc(w[r][q], twB[q * N1 + N2 * r]);
+            lds[Sh::slot(c, g, s, q)] = z;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < N1; ++s) v[s] = lds[Sh::slot(c, g, s, t)];
+    dft<N1, +1, R>(v);
+}
+
+// The round trip of one tile in registers: forward, phase multiply,
+// inverse, then the fused epilogue (nonlinear phase, norm partial sum).
+template <typename R, int N1, int N2, int TS, int G>
+__device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1], const TileGeom& gm,
+                                                const SweepArgs& a, cx<R>* lds, const cx<R>* twA,
+                                                const cx<R>* twB, int c, int g, int t, double& norm_acc)
+{
+    constexpr int RB = N1 / N2;
+    const int flags = a.flags;
+    if (flags & kFlagScale) {
+        const R sc = R(*a.scale);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) v[j] = cscale(v[j], sc);
+    }
+    cx<R> w[RB][N2];
+    forward_ab<R, N1, N2, TS, G>(v, w, lds, twA, c, g, t);
+    const cx<R>* __restrict__ ph =
+        static_cast<const cx<R>*>(a.phase) + (long long)gm.row * a.phase_row_stride + t;
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) w[r][k2] = cmul(w[r][k2], ph[N2 * r + N1 * k2]);
+    inverse_ba<R, N1, N2, TS, G>(w, v, lds, twB, c, g, t);
+    if (flags & kFlagNL) {
+        const R beta = R(a.beta), dt = R(a.dt);
+        const bool imag = flags & kFlagImag;
+#pragma unroll
+        for (int j = 0; j < N1; ++j) nl_phase<R>(v[j], vp[j], beta, dt, imag);
+    }
+    if (flags & kFlagNorm) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            const double re = v[j].re, im = v[j].im;
+            s += re * re + im * im;
+        }
+        if (gm.active) norm_acc += s;
+    }
+}
+
+// One launch over all tiles of an axis, one tile per workgroup; every mode.
 template <typename R, int N1, int N2, int TS, int G, int MODE>
 __global__ __launch_bounds__(TS * G * N2) void k_sweep(const SweepArgs a)
 {
     using Sh = SweepShape<R, N1, N2, TS, G>;
     constexpr int N = Sh::N;
     constexpr int RB = Sh::RB;
+    constexpr int NL = Sh::NL;
     __shared__ cx<R> lds[Sh::LDS];
 
     const int tid = threadIdx.x;
@@ -133,56 +313,55 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep(const SweepArgs a)
     const int t = (tid / TS) % N2;
     const int g = tid / (TS * N2);
     const int L = c + TS * g;
-
-    // ---- line geometry
-    long long base;
-    int row = 0;
-    long long coord_idx = 0;
-    bool active;
-    if constexpr (TS == 1) {
-        const long long line = (long long)blockIdx.x * Sh::NL + L;
-        active = line < a.nlines;
-        base = line * N;
-        if (a.row_mod > 0) row = (int)(line % a.row_mod);
-        if (a.coord_mod > 0) coord_idx = line % a.coord_mod;
-    } else {
-        const int col = blockIdx.x * Sh::NL + L;
-        active = col < a.ncols;
-        base = (long long)col + (long long)blockIdx.y * a.OS;
-        if (a.row_from_col) row = col;
-        coord_idx = col;
-    }
-    const long long S = a.S;
+    const long long tile = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+    const TileGeom gm = tile_geom<N, N2, TS, NL>(a, tile, L, t);
     const int flags = a.flags;
-    const cx<R>* __restrict__ tw = static_cast<const cx<R>*>(a.tw);
+    const cx<R>* __restrict__ twA = static_cast<const cx<R>*>(a.tw) + t;
+    const cx<R>* __restrict__ twB = static_cast<const cx<R>*>(a.tw) + N + t;
     cx<R>* __restrict__ psi = static_cast<cx<R>*>(a.psi);
-    const R* __restrict__ V = static_cast<const R*>(a.V);
+    const long long nblocks = (long long)gridDim.x * gridDim.y;
 
     cx<R> v[N1];
     R vp[N1];
-    double acc[5] = {0, 0, 0, 0, 0};
 
-    if constexpr (MODE != kInverse) {
-        // ---- phase A: load x[N2 j1 + t]
-        if (active) {
-#pragma unroll
-            for (int j = 0; j < N1; ++j) v[j] = psi[base + (long long)(N2 * j + t) * S];
-        } else {
-#pragma unroll
-            for (int j = 0; j < N1; ++j) v[j] = {R(0), R(0)};
+    if constexpr (MODE == kRoundTrip) {
+        load_line<R, N1>(v, psi, gm);
+        if (flags & kFlagNL) load_pot<R, N1>(vp, static_cast<const R*>(a.V), gm);
+        double nacc = 0.0;
+        round_trip_tile<R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
+        if (flags & kFlagNorm) {
+            double acc1[1] = {nacc};
+            block_partials<R, Sh::NT, 1>(acc1, a.partial, nblocks, tile);
         }
-        const bool need_v = (MODE == kRoundTrip && (flags & kFlagNL)) ||
-                            (MODE == kEnergy && (flags & kFlagPointwise));
-        if (need_v) {
+        store_line<R, N1>(v, psi, gm);
+    } else if constexpr (MODE == kForward) {
+        cx<R> w[RB][N2];
+        load_line<R, N1>(v, psi, gm);
+        forward_ab<R, N1, N2, TS, G>(v, w, lds, twA, c, g, t);
+        if (gm.active) {
+            cx<R>* __restrict__ ob = static_cast<cx<R>*>(a.out) + gm.ubase;
 #pragma unroll
-            for (int j = 0; j < N1; ++j) vp[j] = (active && V) ? V[base + (long long)(N2 * j + t) * S] : R(0);
-        }
-        if (MODE == kRoundTrip && (flags & kFlagScale)) {
-            const R sc = R(*a.scale);
+            for (int r = 0; r < RB; ++r)
 #pragma unroll
-            for (int j = 0; j < N1; ++j) v[j] = cscale(v[j], sc);
+                for (int k2 = 0; k2 < N2; ++k2) lane_ref(ob, (N2 * r + N1 * k2) * gm.kstride, gm.lane) = w[r][k2];
         }
-        if (MODE == kEnergy && (flags & kFlagPointwise)) {
+    } else if constexpr (MODE == kInverse) {
+        cx<R> w[RB][N2];
+        const cx<R>* ib = psi + gm.ubase;
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) w[r][k2] = lane_ref(ib, (N2 * r + N1 * k2) * gm.kstride, gm.lane);
+        inverse_ba<R, N1, N2, TS, G>(w, v, lds, twB, c, g, t);
+        const R inv = R(1) / R(N);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) v[j] = cscale(v[j], inv);
+        store_line<R, N1>(v, static_cast<cx<R>*>(a.out), gm);
+    } else {  // kEnergy
+        double acc[5] = {0, 0, 0, 0, 0};
+        load_line<R, N1>(v, psi, gm);
+        if (flags & kFlagPointwise) {
+            load_pot<R, N1>(vp, static_cast<const R*>(a.V), gm);
 #pragma unroll
             for (int j = 0; j < N1; ++j) {
                 const double re = v[j].re, im = v[j].im;
@@ -192,123 +371,88 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep(const SweepArgs a)
                 acc[4] += a2 * a2;
             }
         }
-        dft<N1, -1, R>(v);
-#pragma unroll
-        for (int s = 1; s < N1; ++s) v[s] = cmul(v[s], tw[t * s]);
-#pragma unroll
-        for (int s = 0; s < N1; ++s) lds[Sh::slot(c, g, s, t)] = v[s];
-        __syncthreads();
-    }
-
-    // ---- phase B: rows s = t + N2 r hold X[s + N1 k2]
-    cx<R> w[RB][N2];
-    if constexpr (MODE != kInverse) {
-#pragma unroll
-        for (int r = 0; r < RB; ++r) {
-#pragma unroll
-            for (int q = 0; q < N2; ++q) w[r][q] = lds[Sh::slot(c, g, t + N2 * r, q)];
-            dft<N2, -1, R>(w[r]);
-        }
-    } else {
-        if (active) {
-            const cx<R>* __restrict__ src = static_cast<const cx<R>*>(a.psi);
-#pragma unroll
-            for (int r = 0; r < RB; ++r)
-#pragma unroll
-                for (int k2 = 0; k2 < N2; ++k2) w[r][k2] = src[base + (long long)(t + N2 * r + N1 * k2) * S];
-        } else {
-#pragma unroll
-            for (int r = 0; r < RB; ++r)
-#pragma unroll
-                for (int k2 = 0; k2 < N2; ++k2) w[r][k2] = {R(0), R(0)};
-        }
-    }
-
-    if constexpr (MODE == kForward) {
-        if (active) {
-            cx<R>* __restrict__ dst = static_cast<cx<R>*>(a.out);
-#pragma unroll
-            for (int r = 0; r < RB; ++r)
-#pragma unroll
-                for (int k2 = 0; k2 < N2; ++k2) dst[base + (long long)(t + N2 * r + N1 * k2) * S] = w[r][k2];
-        }
-        return;
-    }
-
-    if constexpr (MODE == kEnergy) {
+        cx<R> w[RB][N2];
+        forward_ab<R, N1, N2, TS, G>(v, w, lds, twA, c, g, t);
+        const double* __restrict__ xi = a.xi + t;
 #pragma unroll
         for (int r = 0; r < RB; ++r)
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2) {
-                const double xi = a.xi[t + N2 * r + N1 * k2];
+                const double x = xi[N2 * r + N1 * k2];
                 const double re = w[r][k2].re, im = w[r][k2].im;
                 const double p = re * re + im * im;
-                acc[0] += xi * xi * p;
-                acc[1] += xi * p;
+                acc[0] += x * x * p;
+                acc[1] += x * p;
             }
-        if (a.coord) acc[1] *= active ? a.coord[coord_idx] : 0.0;
-        const long long nblocks = (long long)gridDim.x * gridDim.y;
-        const long long block = (long long)blockIdx.y * gridDim.x + blockIdx.x;
-        block_partials<R, Sh::NT, 5>(acc, a.partial, nblocks, block);
-        return;
-    }
-
-    if constexpr (MODE == kRoundTrip) {
-        const cx<R>* __restrict__ ph =
-            static_cast<const cx<R>*>(a.phase) + (long long)row * a.phase_row_stride;
+        if (a.coord) acc[1] *= a.coord[gm.coord_idx];
+        if (!gm.active) {
 #pragma unroll
-        for (int r = 0; r < RB; ++r)
-#pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) w[r][k2] = cmul(w[r][k2], ph[t + N2 * r + N1 * k2]);
-    }
-
-    // ---- inverse phase B, conj twiddles, exchange
-#pragma unroll
-    for (int r = 0; r < RB; ++r) {
-        dft<N2, +1, R>(w[r]);
-        const int s = t + N2 * r;
-#pragma unroll
-        for (int q = 0; q < N2; ++q) {
-            const cx<R> z = (q == 0 || s == 0) ? w[r][q] : cmulc(w[r][q], tw[q * s]);
-            lds[Sh::slot(c, g, s, q)] = z;
+            for (int k = 0; k < 5; ++k) acc[k] = 0.0;
         }
+        block_partials<R, Sh::NT, 5>(acc, a.partial, nblocks, tile);
     }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < N1; ++s) v[s] = lds[Sh::slot(c, g, s, t)];
-    dft<N1, +1, R>(v);
+}
 
-    if constexpr (MODE == kInverse) {
-        const R inv = R(1) / R(N);
-#pragma unroll
-        for (int j = 0; j < N1; ++j) v[j] = cscale(v[j], inv);
+// Persistent, software-pipelined round trip: each workgroup walks tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ... and issues the loads of its next
+// tile before transforming the current one, so HBM traffic overlaps the
+// FFT arithmetic within every wavefront.
+template <typename R, int N1, int N2, int TS, int G>
+__global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
+{
+    using Sh = SweepShape<R, N1, N2, TS, G>;
+    constexpr int N = Sh::N;
+    constexpr int NL = Sh::NL;
+    __shared__ cx<R> lds[Sh::LDS];
+
+    const int tid = threadIdx.x;
+    const int c = tid % TS;
+    const int t = (tid / TS) % N2;
+    const int g = tid / (TS * N2);
+    const int L = c + TS * g;
+    const int flags = a.flags;
+    const bool nl = flags & kFlagNL;
+    const cx<R>* __restrict__ twA = static_cast<const cx<R>*>(a.tw) + t;
+    const cx<R>* __restrict__ twB = static_cast<const cx<R>*>(a.tw) + N + t;
+    cx<R>* __restrict__ psi = static_cast<cx<R>*>(a.psi);
+    const R* __restrict__ V = static_cast<const R*>(a.V);
+
+    long long tile = blockIdx.x;
+    double nacc = 0.0;
+    cx<R> v[N1];
+    R vp[N1];
+    TileGeom gm = tile_geom<N, N2, TS, NL>(a, tile, L, t);
+    if (tile < a.ntiles) {
+        load_line<R, N1>(v, psi, gm);
+        if (nl) load_pot<R, N1>(vp, V, gm);
     }
-
-    if constexpr (MODE == kRoundTrip) {
-        if (flags & kFlagNL) {
-            const R beta = R(a.beta), dt = R(a.dt);
-            const bool imag = flags & kFlagImag;
-#pragma unroll
-            for (int j = 0; j < N1; ++j) nl_phase<R>(v[j], vp[j], beta, dt, imag);
+    while (tile < a.ntiles) {
+        const long long next = tile + gridDim.x;
+        const bool more = next < a.ntiles;
+        TileGeom gn;
+        cx<R> vn[N1];
+        R vpn[N1];
+        if (more) {
+            gn = tile_geom<N, N2, TS, NL>(a, next, L, t);
+            load_line<R, N1>(vn, psi, gn);
+            if (nl) load_pot<R, N1>(vpn, V, gn);
         }
-        if (flags & kFlagNorm) {
-            double s = 0.0;
+        round_trip_tile<R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
+        store_line<R, N1>(v, psi, gm);
+        if (!more) break;
+        __syncthreads();  // every lane is done with the exchange buffer
 #pragma unroll
-            for (int j = 0; j < N1; ++j) {
-                const double re = v[j].re, im = v[j].im;
-                s += re * re + im * im;
-            }
-            double acc1[1] = {active ? s : 0.0};
-            const long long nblocks = (long long)gridDim.x * gridDim.y;
-            const long long block = (long long)blockIdx.y * gridDim.x + blockIdx.x;
-            block_partials<R, Sh::NT, 1>(acc1, a.partial, nblocks, block);
+        for (int j = 0; j < N1; ++j) v[j] = vn[j];
+        if (nl) {
+#pragma unroll
+            for (int j = 0; j < N1; ++j) vp[j] = vpn[j];
         }
+        gm = gn;
+        tile = next;
     }
-
-    if (active) {
-        cx<R>* __restrict__ dst = static_cast<cx<R>*>(MODE == kInverse ? a.out : a.psi);
-#pragma unroll
-        for (int j = 0; j < N1; ++j) dst[base + (long long)(N2 * j + t) * S] = v[j];
+    if (flags & kFlagNorm) {
+        double acc1[1] = {nacc};
+        block_partials<R, Sh::NT, 1>(acc1, a.partial, gridDim.x, blockIdx.x);
     }
 }
 

@@ -1,0 +1,201 @@
+// sfd_variants.hpp -- the compiled launch variants of k_sweep and their
+// dispatch.  A variant fixes the line split N = N1*N2 and the workgroup
+// shape: TS x-columns per lane group (1 = contiguous x axis) and G groups,
+// i.e. NL = TS*G lines and NL*N2 threads per workgroup.  The autotuner in
+// sfd.hip times every variant of an axis once per context and keeps the
+// fastest; index 0 is the default.
+#pragma once
+#include "sfd_sweep.hpp"
+
+namespace sfd {
+
+struct VariantInfo {
+    int n1, n2, ts, g, pipe;
+};
+
+// pipe = 1: the round trip runs as the persistent, software-pipelined
+// k_sweep_pipe (the other modes use k_sweep with the same shape).
+template <int N1_, int N2_, int TS_, int G_, int P_ = 0>
+struct V {
+    static constexpr int N1 = N1_, N2 = N2_, TS = TS_, G = G_, P = P_;
+};
+template <typename... Vs>
+struct VL {
+    static constexpr int size = sizeof...(Vs);
+};
+
+// Per (precision, N): contiguous (x axis) and strided (y/z axes) variants.
+// Every variant keeps its exchange buffer within 64 KiB of LDS and at least
+// one full wavefront per workgroup.
+template <typename R, int N>
+struct Variants;
+
+// the lists are passed parenthesised, so that their commas survive the macro
+template <typename T>
+struct Unparen;
+template <typename T>
+struct Unparen<void(T)> {
+    using type = T;
+};
+
+#define SFD_VARIANTS(R_, N_, CONTIG, STRIDED)                  \
+    template <>                                                \
+    struct Variants<R_, N_> {                                  \
+        using contig = typename Unparen<void CONTIG>::type;    \
+        using strided = typename Unparen<void STRIDED>::type;  \
+    };
+
+// ---------------------------------------------------------------- FP32
+SFD_VARIANTS(float, 2, (VL<V<2, 1, 1, 256>>), (VL<V<2, 1, 16, 4>>))
+SFD_VARIANTS(float, 4, (VL<V<2, 2, 1, 128>>), (VL<V<2, 2, 16, 2>>))
+SFD_VARIANTS(float, 8, (VL<V<4, 2, 1, 128>>), (VL<V<4, 2, 16, 2>>))
+SFD_VARIANTS(float, 16, (VL<V<4, 4, 1, 64>>), (VL<V<4, 4, 16, 1>>))
+SFD_VARIANTS(float, 32, (VL<V<8, 4, 1, 64>>), (VL<V<8, 4, 16, 1>>))
+SFD_VARIANTS(float, 64, (VL<V<8, 8, 1, 32>, V<16, 4, 1, 64>>), (VL<V<8, 8, 16, 1>, V<8, 8, 32, 1>>))
+SFD_VARIANTS(float, 128, (VL<V<16, 8, 1, 32>, V<16, 8, 1, 16>, V<32, 4, 1, 32>, V<16, 8, 1, 16, 1>>),
+             (VL<V<16, 8, 16, 1>, V<16, 8, 32, 1>, V<32, 4, 32, 1>, V<16, 8, 16, 1, 1>>))
+SFD_VARIANTS(float, 256,
+             (VL<V<16, 16, 1, 16>, V<16, 16, 1, 8>, V<32, 8, 1, 16>, V<32, 8, 1, 8>, V<16, 16, 1, 16, 1>,
+                 V<16, 16, 1, 8, 1>>),
+             (VL<V<16, 16, 16, 1>, V<16, 16, 32, 1>, V<32, 8, 16, 1>, V<32, 8, 32, 1>, V<16, 16, 16, 1, 1>,
+                 V<16, 16, 8, 1, 1>>))
+SFD_VARIANTS(float, 512, (VL<V<32, 16, 1, 8>, V<32, 16, 1, 4>, V<32, 16, 1, 4, 1>>),
+             (VL<V<32, 16, 16, 1>, V<32, 16, 8, 1>, V<32, 16, 8, 1, 1>>))
+SFD_VARIANTS(float, 1024, (VL<V<32, 32, 1, 4>, V<32, 32, 1, 2>>), (VL<V<32, 32, 8, 1>, V<32, 32, 4, 1>>))
+
+// ---------------------------------------------------------------- FP64
+SFD_VARIANTS(double, 2, (VL<V<2, 1, 1, 256>>), (VL<V<2, 1, 16, 4>>))
+SFD_VARIANTS(double, 4, (VL<V<2, 2, 1, 128>>), (VL<V<2, 2, 16, 2>>))
+SFD_VARIANTS(double, 8, (VL<V<4, 2, 1, 128>>), (VL<V<4, 2, 16, 2>>))
+SFD_VARIANTS(double, 16, (VL<V<4, 4, 1, 64>>), (VL<V<4, 4, 16, 1>>))
+SFD_VARIANTS(double, 32, (VL<V<8, 4, 1, 64>>), (VL<V<8, 4, 16, 1>>))
+SFD_VARIANTS(double, 64, (VL<V<8, 8, 1, 32>, V<16, 4, 1, 32>>), (VL<V<8, 8, 16, 1>, V<8, 8, 32, 1>>))
+SFD_VARIANTS(double, 128, (VL<V<16, 8, 1, 16>, V<16, 8, 1, 8>, V<32, 4, 1, 16>>),
+             (VL<V<16, 8, 16, 1>, V<16, 8, 32, 1>, V<32, 4, 16, 1>>))
+SFD_VARIANTS(double, 256, (VL<V<16, 16, 1, 8>, V<16, 16, 1, 4>, V<16, 16, 1, 8, 1>, V<16, 16, 1, 4, 1>>),
+             (VL<V<16, 16, 16, 1>, V<16, 16, 8, 1>, V<16, 16, 16, 1, 1>, V<16, 16, 8, 1, 1>>))
+SFD_VARIANTS(double, 512, (VL<V<32, 16, 1, 4>>), (VL<V<32, 16, 8, 1>, V<32, 16, 4, 1>>))
+SFD_VARIANTS(double, 1024, (VL<V<32, 32, 1, 2>>), (VL<V<32, 32, 4, 1>, V<32, 32, 2, 1>>))
+
+#undef SFD_VARIANTS
+
+template <typename R, int MODE, typename L>
+struct Dispatch;
+
+template <typename R, int MODE>
+struct Dispatch<R, MODE, VL<>> {
+    static bool info(int, VariantInfo*) { return false; }
+    static bool launch(int, dim3, dim3, hipStream_t, const SweepArgs&) { return false; }
+    static int occupancy(int) { return 0; }
+};
+
+template <typename R, int MODE, typename V0, typename... Vs>
+struct Dispatch<R, MODE, VL<V0, Vs...>> {
+    static bool info(int idx, VariantInfo* out)
+    {
+        if (idx == 0) {
+            *out = {V0::N1, V0::N2, V0::TS, V0::G, V0::P};
+            return true;
+        }
+        return Dispatch<R, MODE, VL<Vs...>>::info(idx - 1, out);
+    }
+    static bool launch(int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a)
+    {
+        if (idx == 0) {
+            if constexpr (V0::P && MODE == kRoundTrip)
+                hipLaunchKernelGGL((k_sweep_pipe<R, V0::N1, V0::N2, V0::TS, V0::G>), grid, block, 0, s, a);
+            else
+                hipLaunchKernelGGL((k_sweep<R, V0::N1, V0::N2, V0::TS, V0::G, MODE>), grid, block, 0, s, a);
+            return true;
+        }
+        return Dispatch<R, MODE, VL<Vs...>>::launch(idx - 1, grid, block, s, a);
+    }
+    // resident workgroups per CU of the persistent round-trip kernel
+    static int occupancy(int idx)
+    {
+        if (idx == 0) {
+            if constexpr (V0::P) {
+                int nb = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &nb, reinterpret_cast<const void*>(&k_sweep_pipe<R, V0::N1, V0::N2, V0::TS, V0::G>),
+                        V0::TS * V0::G * V0::N2, 0) != hipSuccess)
+                    return 1;
+                return nb > 0 ? nb : 1;
+            }
+            return 0;
+        }
+        return Dispatch<R, MODE, VL<Vs...>>::occupancy(idx - 1);
+    }
+};
+
+// Runtime lookup by line length.
+template <typename R, int N>
+int variant_count(bool contig)
+{
+    return contig ? Variants<R, N>::contig::size : Variants<R, N>::strided::size;
+}
+
+template <typename R, int MODE, int N>
+bool launch_variant_n(bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a)
+{
+    if (contig) return Dispatch<R, MODE, typename Variants<R, N>::contig>::launch(idx, grid, block, s, a);
+    return Dispatch<R, MODE, typename Variants<R, N>::strided>::launch(idx, grid, block, s, a);
+}
+
+template <typename R, int N>
+bool info_variant_n(bool contig, int idx, VariantInfo* out)
+{
+    if (contig) return Dispatch<R, 0, typename Variants<R, N>::contig>::info(idx, out);
+    return Dispatch<R, 0, typename Variants<R, N>::strided>::info(idx, out);
+}
+
+#define SFD_FOR_EACH_N(M) M(2) M(4) M(8) M(16) M(32) M(64) M(128) M(256) M(512) M(1024)
+
+template <typename R>
+int n_variants(int n, bool contig)
+{
+#define SFD_CASE(N_) \
+    if (n == N_) return variant_count<R, N_>(contig);
+    SFD_FOR_EACH_N(SFD_CASE)
+#undef SFD_CASE
+    return 0;
+}
+
+template <typename R, int N>
+int occupancy_variant_n(bool contig, int idx)
+{
+    if (contig) return Dispatch<R, 0, typename Variants<R, N>::contig>::occupancy(idx);
+    return Dispatch<R, 0, typename Variants<R, N>::strided>::occupancy(idx);
+}
+
+template <typename R>
+int variant_occupancy(int n, bool contig, int idx)
+{
+#define SFD_CASE(N_) \
+    if (n == N_) return occupancy_variant_n<R, N_>(contig, idx);
+    SFD_FOR_EACH_N(SFD_CASE)
+#undef SFD_CASE
+    return 0;
+}
+
+template <typename R>
+bool variant_info(int n, bool contig, int idx, VariantInfo* out)
+{
+#define SFD_CASE(N_) \
+    if (n == N_) return info_variant_n<R, N_>(contig, idx, out);
+    SFD_FOR_EACH_N(SFD_CASE)
+#undef SFD_CASE
+    return false;
+}
+
+template <typename R, int MODE>
+bool launch_variant(int n, bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a)
+{
+#define SFD_CASE(N_) \
+    if (n == N_) return launch_variant_n<R, MODE, N_>(contig, idx, grid, block, s, a);
+    SFD_FOR_EACH_N(SFD_CASE)
+#undef SFD_CASE
+    return false;
+}
+
+}  // namespace sfd

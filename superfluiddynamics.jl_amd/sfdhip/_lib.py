@@ -32,6 +32,7 @@ EXPORTS = (
     "sfd_ctx_create", "sfd_ctx_destroy", "sfd_ctx_set_stream", "sfd_split_step",
     "sfd_lap_rot", "sfd_lap_rot_axis", "sfd_solve_nl", "sfd_norm2", "sfd_normalize", "sfd_energy", "sfd_fft_axis", "sfd_alloc", "sfd_free",
     "sfd_upload", "sfd_download", "sfd_synchronize", "sfd_last_error", "sfd_version",
+    "sfd_autotune", "sfd_ctx_get_variant", "sfd_ctx_set_variant",
 )
 
 
@@ -91,6 +92,10 @@ def load():
         "sfd_synchronize": ([vp], c_int),
         "sfd_last_error": ([], ctypes.c_char_p),
         "sfd_version": ([], c_int),
+        "sfd_autotune": ([vp, vp, ctypes.POINTER(ctypes.c_float)], c_int),
+        "sfd_ctx_get_variant": ([vp, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
+                                c_int),
+        "sfd_ctx_set_variant": ([vp, c_int, c_int], c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

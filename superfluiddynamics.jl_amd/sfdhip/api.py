@@ -138,6 +138,24 @@ class _Ctx:
         L.check(lib.sfd_ctx_create(ctypes.byref(h), nd, n, lo, hi, prec, ctypes.c_void_p(stream)))
         self.h = h
 
+    def autotune(self, scratch=None):
+        """Time every compiled sweep variant per axis and keep the fastest
+        (sfd_autotune).  Returns {axis: [ms per launch of each variant]}."""
+        times = (ctypes.c_float * 24)()
+        ptr = ctypes.c_void_p(scratch.data_ptr()) if scratch is not None else None
+        L.check(self.lib.sfd_autotune(self.h, ptr, times))
+        return {a: [t for t in times[a * 8:(a + 1) * 8] if t >= 0] for a in range(3) if times[a * 8] >= 0}
+
+    def variant(self, axis: int):
+        """(index, count, (n1, n2, ts, g, pipelined)) of the sweep variant used on axis."""
+        idx, cnt = ctypes.c_int(), ctypes.c_int()
+        shp = (ctypes.c_int * 5)()
+        L.check(self.lib.sfd_ctx_get_variant(self.h, axis, ctypes.byref(idx), ctypes.byref(cnt), shp))
+        return idx.value, cnt.value, tuple(shp)
+
+    def set_variant(self, axis: int, index: int):
+        L.check(self.lib.sfd_ctx_set_variant(self.h, axis, index))
+
     def sync_stream(self):
         L.check(self.lib.sfd_ctx_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
@@ -184,6 +202,11 @@ class Field:
             self._ctx = _Ctx(self.g)
         self._ctx.sync_stream()
         return self._ctx
+
+    def autotune(self):
+        """Pick the fastest sweep variant per axis for this field's grid,
+        timed on a scratch field (ϕ is not touched)."""
+        return self.ctx().autotune()
 
     def coords(self):
         """Broadcast-shaped FP64 numpy coordinates (x, y[, z]) like ``field.x``."""
@@ -348,6 +371,21 @@ class NumModelSplitImag(_SplitBase):
 def timeStep(n: _SplitBase) -> int:
     """timeStep!(n): one split step in place; returns 1 (Split1) or 2 (Split2)."""
     return n.steps(1)
+
+
+def solveLapRot(n: _SplitBase, Δtl: float) -> None:
+    """solveLapRot!(n, Δtl) (splitting.jl:156-203): the linear flow over Δtl,
+    x, y[, z] round trips (imaginary-time models use real exponents)."""
+    c = n.f.ctx()
+    p = n.param
+    L.check(c.lib.sfd_lap_rot(c.h, ctypes.c_void_p(n.f.ϕ.data_ptr()), p.coeffΔ, p.Ω, float(Δtl), n.mode))
+
+
+def solveNL(n: _SplitBase, Δtl: float) -> None:
+    """solveNL!(n, Δtl) (splitting.jl:205-209): ϕ ← exp(-i(V + β|ϕ|²)Δtl) ϕ."""
+    c = n.f.ctx()
+    p = n.param
+    L.check(c.lib.sfd_solve_nl(c.h, ctypes.c_void_p(n.f.ϕ.data_ptr()), _vptr(p), p.β, float(Δtl), n.mode))
 
 
 def energy(n: _SplitBase, showEnergy: bool = False):

@@ -119,3 +119,45 @@ def test_error_paths(gpu):
     assert b"order" in lib.sfd_last_error()
     with pytest.raises(gpu.ShapeError):
         gpu.Field(gpu.Grid((48, 32), ((-1, 1), (-1, 1)))).ctx()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_nonlinear_phase_accuracy(gpu, prec):
+    """solveNL! on ψ = 1 with V = θ/Δt gives exp(-iθ): the device sine/cosine
+    (Cody-Waite reduction + Taylor series, library fallback for huge |θ|)
+    against numpy over |θ| up to 3e4, multiples of π/2 included."""
+    FT = np.float64 if prec == "f64" else np.float32
+    rng = np.random.default_rng(0)
+    th = np.concatenate([rng.uniform(-4, 4, 4096), rng.uniform(-200, 200, 4096), rng.uniform(-3e4, 3e4, 2048),
+                         (np.arange(-1024, 1024) * (np.pi / 2)), np.linspace(-1e-3, 1e-3, 4096)])
+    th = th.reshape(128, 128)
+    th = th.astype(FT).astype(np.float64)  # representable in the field precision
+    g = gpu.Grid((128, 128), ((0, 1), (0, 1)), FT=FT)
+    f = gpu.Field(g)
+    f.set(np.ones((128, 128), dtype=np.complex128))
+    dt = 0.5
+    pot = gpu.PotentialArray(f, th / dt)
+    p = gpu.GrossPitaevskiiParameters(pot=pot, coeffΔ=-0.5, β=0.0)
+    m = gpu.NumModelSplit2(f, p, dt, 1, 1)
+    gpu.solveNL(m, dt)
+    got = f.to_numpy()
+    exact = np.exp(-1j * (th / dt).astype(FT).astype(np.float64) * dt)
+    err = np.max(np.abs(got - exact))
+    print(f"{prec}: max |error| {err:.3e}")
+    assert err < (4e-16 if prec == "f64" else 3e-7)
+
+
+def test_lap_rot_and_nl_compose_to_split2(gpu, orc):
+    """solveLapRot!(Δt/2) ∘ solveNL!(Δt) ∘ solveLapRot!(Δt/2) equals one timeStep!."""
+    shape, bounds = (64, 32, 16), ((-4, 4), (-3, 3), (-2, 2))
+    psi = smooth_state(shape, bounds, seed=2)
+    V = random_potential(shape, bounds)
+    kw = dict(V=V, coeffD=-0.5, beta=4.0, Omega=0.3, dt=3e-3, nsteps=1, order=2)
+    ref = orc.split_step(psi.copy(), bounds, **kw)
+    f = _field(gpu, psi, bounds)
+    p = gpu.GrossPitaevskiiParameters(pot=gpu.PotentialArray(f, V), coeffΔ=-0.5, β=4.0, Ω=0.3)
+    m = gpu.NumModelSplit2(f, p, 3e-3, 1, 1)
+    gpu.solveLapRot(m, 1.5e-3)
+    gpu.solveNL(m, 3e-3)
+    gpu.solveLapRot(m, 1.5e-3)
+    assert rel_l2(f.to_numpy(), ref) < 1e-12

@@ -93,3 +93,65 @@ def test_thomas_fermi_1000_steps_f64(gpu, orc):
     err = rel_l2(out, ref)
     print(f"TF 128² 1000 steps FP64: rel L2 = {err:.3e}")
     assert err <= 1e-12
+
+
+VARIANT_CASES = [
+    ((64, 64), ((-6, 6), (-6, 6))),
+    ((128, 64, 16), ((-4, 4), (-3, 3), (-2, 2))),
+    ((256, 256), ((-8, 8), (-8, 8))),
+    ((16, 256, 128), ((-2, 2), (-6, 6), (-5, 5))),
+    ((512, 32), ((-8, 8), (-2, 2))),
+    ((32, 1024), ((-2, 2), (-10, 10))),
+]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("case", VARIANT_CASES, ids=lambda c: "x".join(map(str, c[0])))
+def test_every_launch_variant(gpu, orc, case, prec):
+    """Every compiled sweep variant (line split, workgroup shape) of every
+    axis gives the oracle's answer: a Split2 step with V and rotation."""
+    shape, bounds = case
+    psi = smooth_state(shape, bounds, seed=3)
+    V = random_potential(shape, bounds)
+    kw = dict(V=V, coeffD=-0.5, beta=3.0, Omega=0.4, dt=2e-3, nsteps=2, order=2)
+    ref = orc.split_step(psi.copy(), bounds, **kw)
+    FT = np.float64 if prec == "f64" else np.float32
+    g = gpu.Grid(shape, bounds, FT=FT)
+    f = gpu.Field(g)
+    pot = gpu.PotentialArray(f, V)
+    p = gpu.GrossPitaevskiiParameters(pot=pot, coeffΔ=-0.5, β=3.0, Ω=0.4)
+    m = gpu.NumModelSplit2(f, p, 2e-3, 2, 2)
+    mi = gpu.NumModelSplitImag(f, p, 2e-3, 2, 2, order=2)
+    ref_imag = orc.split_step(psi.copy(), bounds, imag=True, **kw)
+    ctx = f.ctx()
+    counts = [ctx.variant(a)[1] for a in range(len(shape))]
+    for a in range(len(shape)):
+        for v in range(counts[a]):
+            for b in range(len(shape)):
+                ctx.set_variant(b, 0)
+            ctx.set_variant(a, v)
+            f.set(psi)
+            m.steps(2)
+            err = rel_l2(f.to_numpy(), ref)
+            f.set(psi)
+            mi.steps(2)
+            erri = rel_l2(f.to_numpy(), ref_imag)
+            print(f"{shape} {prec} axis {a} variant {v} {ctx.variant(a)[2]}: {err:.3e} imag {erri:.3e}")
+            assert err < (F64_TOL if prec == "f64" else F32_TOL)
+            assert erri < (F64_TOL if prec == "f64" else F32_TOL)
+
+
+def test_autotune_keeps_results(gpu, orc):
+    shape, bounds = (128, 64, 32), ((-4, 4), (-3, 3), (-2, 2))
+    psi = smooth_state(shape, bounds, seed=4)
+    kw = dict(coeffD=-0.5, beta=2.0, dt=2e-3, nsteps=3, order=2)
+    ref = orc.split_step(psi.copy(), bounds, **kw)
+    g = gpu.Grid(shape, bounds, FT=np.float32)
+    f = gpu.Field(g)
+    f.set(psi)
+    times = f.autotune()
+    assert set(times) == {0, 1, 2}
+    assert all(len(v) >= 1 and all(t > 0 for t in v) for v in times.values())
+    p = gpu.GrossPitaevskiiParameters(pot=gpu.PotentialZero(f), coeffΔ=-0.5, β=2.0)
+    gpu.NumModelSplit2(f, p, 2e-3, 3, 3).steps(3)
+    assert rel_l2(f.to_numpy(), ref) < F32_TOL
