@@ -32,8 +32,9 @@ extern "C" {
 #define SFD_ERR_OOM (-4)   /* device allocation failed                     */
 #define SFD_ERR_HIP (-5)   /* a HIP runtime call or kernel launch failed   */
 
-#define SFD_F32 1 /* complex FP32 field, FP32 V, FP64 reductions */
-#define SFD_F64 2 /* complex FP64 field, FP64 V                  */
+#define SFD_F32 1  /* complex FP32 field and arithmetic, FP32 V, FP64 reductions */
+#define SFD_F64 2  /* complex FP64 field and arithmetic, FP64 V                  */
+#define SFD_F32X 3 /* complex FP32 field and FP32 V in memory, FP64 arithmetic   */
 
 #define SFD_REAL_TIME 0 /* exp(i ...) flows: NumModelSplit1/2                */
 #define SFD_IMAG_TIME 1 /* exp(...) flows + renormalisation after every step */

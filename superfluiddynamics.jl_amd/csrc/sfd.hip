@@ -51,18 +51,18 @@ int fail(int code, const char* fmt, ...)
 constexpr int lg2(int n) { return n <= 1 ? 0 : 1 + lg2(n / 2); }
 
 // ----------------------------------------------------------- small kernels
-template <typename R>
-__global__ void k_norm2(const cx<R>* __restrict__ psi, long long n, double* partial)
+template <typename S>
+__global__ void k_norm2(const cx<S>* __restrict__ psi, long long n, double* partial)
 {
     double s = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
-        const cx<R> v = psi[i];
+        const cx<S> v = psi[i];
         const double re = v.re, im = v.im;
         s += re * re + im * im;
     }
     double acc[1] = {s};
-    block_partials<R, 256, 1>(acc, partial, gridDim.x, blockIdx.x);
+    block_partials<S, 256, 1>(acc, partial, gridDim.x, blockIdx.x);
 }
 
 // out[k] = sum_b partial[k*nb + b] in a fixed order (deterministic).
@@ -86,31 +86,32 @@ __global__ void k_finalize(const double* __restrict__ partial, long long nb, int
     if (scale_out && threadIdx.x == 0) scale_out[0] = 1.0 / sqrt(out[0] * dv);
 }
 
-// psi *= s (divide = 0) or psi /= s (divide = 1); s is read from device
-// memory when s_dev is not null.
-template <typename R>
-__global__ void k_scale(cx<R>* __restrict__ psi, long long n, double s_host, const double* s_dev, int divide)
+// psi *= s (divide = 0) or psi /= s (divide = 1) in FP64, rounded once to
+// the storage precision; s is read from device memory when s_dev is not null.
+template <typename S>
+__global__ void k_scale(cx<S>* __restrict__ psi, long long n, double s_host, const double* s_dev, int divide)
 {
     const double s = s_dev ? *s_dev : s_host;
-    const R f = R(s);
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
-        cx<R> v = psi[i];
-        if (divide) { v.re /= f; v.im /= f; }
-        else        { v.re *= f; v.im *= f; }
-        psi[i] = v;
+        const cx<S> v = psi[i];
+        double re = v.re, im = v.im;
+        if (divide) { re /= s; im /= s; }
+        else        { re *= s; im *= s; }
+        psi[i] = {S(re), S(im)};
     }
 }
 
-// Pointwise nonlinear flow (solveNL!, splitting.jl:205-209).
-template <typename R>
-__global__ void k_nl(cx<R>* __restrict__ psi, const R* __restrict__ V, long long n, R beta, R dt, int imag)
+// Pointwise nonlinear flow (solveNL!, splitting.jl:205-209): storage S,
+// arithmetic R.
+template <typename S, typename R>
+__global__ void k_nl(cx<S>* __restrict__ psi, const S* __restrict__ V, long long n, R beta, R dt, int imag)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
-        cx<R> v = psi[i];
-        nl_phase<R>(v, V ? V[i] : R(0), beta, dt, imag != 0);
-        psi[i] = v;
+        cx<R> v = widen<R>(psi[i]);
+        nl_phase<R>(v, V ? R(V[i]) : R(0), beta, dt, imag != 0);
+        psi[i] = narrow<S>(v);
     }
 }
 
@@ -163,7 +164,7 @@ struct sfd_ctx {
     double* scal = nullptr;     // [0] pending scale, [1..31] sums
     std::vector<PhaseEntry> phases;
 
-    size_t csize() const { return prec == SFD_F32 ? 8 : 16; }
+    size_t csize() const { return prec == SFD_F64 ? 16 : 8; }
     double dvol() const
     {
         double v = 1.0;
@@ -205,9 +206,13 @@ int upload_sync(void* dst, const void* src, size_t bytes)
 //   z: exp(s  cD xi^2 tau)
 // s = i (real time) or 1 (imaginary time).  Formed in FP64 with the
 // reference's operation order, rounded once to the field precision.
-template <typename R>
+template <typename P>
 int build_phase(sfd_ctx* c, int axis, double tau, double cD, double Om, int imag, void** out, int* rows_out)
 {
+    using R = typename P::R;
+    using S = typename P::S;
+    (void)sizeof(R);
+    (void)sizeof(S);
     for (auto& p : c->phases)
         if (p.axis == axis && p.tau == tau && p.cD == cD && p.Om == Om && p.imag == imag) {
             *out = p.dev;
@@ -301,12 +306,16 @@ Launch sweep_geometry(const sfd_ctx* c, int axis, SweepArgs& a, bool round_trip)
     return L;
 }
 
-template <typename R, int MODE>
+template <typename P, int MODE>
 int launch_sweep(sfd_ctx* c, int axis, SweepArgs a)
 {
+    using R = typename P::R;
+    using S = typename P::S;
+    (void)sizeof(R);
+    (void)sizeof(S);
     Launch L = sweep_geometry(c, axis, a, MODE == kRoundTrip);
     a.tw = c->tw[axis];
-    if (!launch_variant<R, MODE>(c->n[axis], axis == 0, c->var[axis], L.grid, L.block, c->stream, a))
+    if (!launch_variant<P, MODE>(c->n[axis], axis == 0, c->var[axis], L.grid, L.block, c->stream, a))
         return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", c->var[axis], c->n[axis]);
     SFD_HIP(hipGetLastError());
     return SFD_OK;
@@ -320,15 +329,19 @@ long long sweep_nblocks(const sfd_ctx* c, int axis, bool round_trip = true)
 
 // Select launch variant `idx` for `axis` and (re)build the twiddle table of
 // its line split.
-template <typename R>
+template <typename P>
 int set_variant(sfd_ctx* c, int axis, int idx)
 {
+    using R = typename P::R;
+    using S = typename P::S;
+    (void)sizeof(R);
+    (void)sizeof(S);
     const int n = c->n[axis];
     VariantInfo vi;
-    if (!variant_info<R>(n, axis == 0, idx, &vi))
+    if (!variant_info<P>(n, axis == 0, idx, &vi))
         return fail(SFD_ERR_ARG, "axis %d: variant %d does not exist (%d variants)", axis, idx,
-                    n_variants<R>(n, axis == 0));
-    c->pipe_occ[axis] = vi.pipe ? variant_occupancy<R>(n, axis == 0, idx) : 0;
+                    n_variants<P>(n, axis == 0));
+    c->pipe_occ[axis] = vi.pipe ? variant_occupancy<P>(n, axis == 0, idx) : 0;
     if (c->tw[axis] && c->vi[axis].n1 == vi.n1) {
         c->var[axis] = idx;
         c->vi[axis] = vi;
@@ -348,23 +361,31 @@ int set_variant(sfd_ctx* c, int axis, int idx)
     return SFD_OK;
 }
 
-template <typename R>
+template <typename P>
 int base_args(sfd_ctx* c, int axis, SweepArgs& a)
 {
+    using R = typename P::R;
+    using S = typename P::S;
+    (void)sizeof(R);
+    (void)sizeof(S);
     std::memset(&a, 0, sizeof a);
     return SFD_OK;
 }
 
 // One linear round trip along `axis` over tau, with optional fused flags.
-template <typename R>
+template <typename P>
 int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, double cD, double Om, int imag,
                int flags, double beta, double dt)
 {
+    using R = typename P::R;
+    using S = typename P::S;
+    (void)sizeof(R);
+    (void)sizeof(S);
     SweepArgs a;
-    base_args<R>(c, axis, a);
+    base_args<P>(c, axis, a);
     void* ph = nullptr;
     int rows = 1;
-    SFD_TRY(build_phase<R>(c, axis, tau, cD, Om, imag, &ph, &rows));
+    SFD_TRY(build_phase<P>(c, axis, tau, cD, Om, imag, &ph, &rows));
     a.psi = psi;
     a.V = V;
     a.phase = ph;
@@ -379,7 +400,7 @@ int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, doubl
         SFD_TRY(ensure_partial(c, 5 * sweep_nblocks(c, axis)));
         a.partial = c->partial;
     }
-    return launch_sweep<R, kRoundTrip>(c, axis, a);
+    return launch_sweep<P, kRoundTrip>(c, axis, a);
 }
 
 int check_ctx(sfd_ctx* c)
@@ -395,13 +416,17 @@ long long grid_blocks(long long n)
     return b < 2048 ? (b < 1 ? 1 : b) : 2048;
 }
 
-template <typename R>
+template <typename P>
 int norm2_impl(sfd_ctx* c, const void* psi, double* dev_out)
 {
+    using R = typename P::R;
+    using S = typename P::S;
+    (void)sizeof(R);
+    (void)sizeof(S);
     const long long nb = grid_blocks(c->nsite);
     SFD_TRY(ensure_partial(c, nb));
-    hipLaunchKernelGGL(k_norm2<R>, dim3((unsigned)nb), dim3(256), 0, c->stream,
-                       static_cast<const cx<R>*>(psi), c->nsite, c->partial);
+    hipLaunchKernelGGL(k_norm2<S>, dim3((unsigned)nb), dim3(256), 0, c->stream,
+                       static_cast<const cx<S>*>(psi), c->nsite, c->partial);
     SFD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nb, 1, dev_out, 0.0,
                        (double*)nullptr);
@@ -409,10 +434,14 @@ int norm2_impl(sfd_ctx* c, const void* psi, double* dev_out)
     return SFD_OK;
 }
 
-template <typename R>
+template <typename P>
 int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt,
                int64_t nsteps, int order, int mode)
 {
+    using R = typename P::R;
+    using S = typename P::S;
+    (void)sizeof(R);
+    (void)sizeof(S);
     const int imag = mode == SFD_IMAG_TIME;
     const int D = c->ndim;
     const double tau = order == 2 ? dt * 0.5 : dt;
@@ -427,7 +456,7 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
                 if (ax == 0 && h == 0 && pending) flags |= kFlagScale;
                 if (ax == D - 1 && h == 0) flags |= nlflags;
                 if (ax == D - 1 && h == order - 1 && imag) flags |= kFlagNorm;
-                SFD_TRY(round_trip<R>(c, psi, V, ax, tau, cD, Om, imag, flags, beta, dt));
+                SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, flags, beta, dt));
             }
         }
         if (imag) {
@@ -440,20 +469,24 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
     }
     if (pending) {
         const long long nb = grid_blocks(c->nsite);
-        hipLaunchKernelGGL(k_scale<R>, dim3((unsigned)nb), dim3(256), 0, c->stream, static_cast<cx<R>*>(psi),
+        hipLaunchKernelGGL(k_scale<S>, dim3((unsigned)nb), dim3(256), 0, c->stream, static_cast<cx<S>*>(psi),
                            c->nsite, 0.0, (const double*)c->scal, 0);
         SFD_HIP(hipGetLastError());
     }
     return SFD_OK;
 }
 
-template <typename R>
+template <typename P>
 int energy_impl(sfd_ctx* c, const void* psi, const void* V, double cD, double beta, double Om, double* out5)
 {
+    using R = typename P::R;
+    using S = typename P::S;
+    (void)sizeof(R);
+    (void)sizeof(S);
     double sums[3][5];
     for (int ax = 0; ax < c->ndim; ++ax) {
         SweepArgs a;
-        base_args<R>(c, ax, a);
+        base_args<P>(c, ax, a);
         a.psi = const_cast<void*>(psi);
         a.V = V;
         a.xi = c->xi[ax];
@@ -463,7 +496,7 @@ int energy_impl(sfd_ctx* c, const void* psi, const void* V, double cD, double be
         const long long nb = sweep_nblocks(c, ax, false);
         SFD_TRY(ensure_partial(c, 5 * nb));
         a.partial = c->partial;
-        SFD_TRY(launch_sweep<R, kEnergy>(c, ax, a));
+        SFD_TRY(launch_sweep<P, kEnergy>(c, ax, a));
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nb, 5, c->scal + 8, 0.0,
                            (double*)nullptr);
         SFD_HIP(hipGetLastError());
@@ -488,28 +521,36 @@ int energy_impl(sfd_ctx* c, const void* psi, const void* V, double cD, double be
     return SFD_OK;
 }
 
-template <typename R>
+template <typename P>
 int fft_impl(sfd_ctx* c, const void* in, void* out, int axis, int dir)
 {
+    using R = typename P::R;
+    using S = typename P::S;
+    (void)sizeof(R);
+    (void)sizeof(S);
     SweepArgs a;
-    base_args<R>(c, axis, a);
+    base_args<P>(c, axis, a);
     a.psi = const_cast<void*>(in);
     a.out = out;
-    if (dir == SFD_FORWARD) return launch_sweep<R, kForward>(c, axis, a);
-    return launch_sweep<R, kInverse>(c, axis, a);
+    if (dir == SFD_FORWARD) return launch_sweep<P, kForward>(c, axis, a);
+    return launch_sweep<P, kInverse>(c, axis, a);
 }
 
 // Autotuner: for every axis with more than one compiled variant, time the
 // identity round trip (tau = 0) of each variant on a scratch field and keep
 // the fastest.  Each candidate gets 2 warm-up launches, then the best of 3
 // timed batches of 8 launches (HIP events on the context's stream).
-template <typename R>
+template <typename P>
 int autotune_impl(sfd_ctx* c, void* scratch, float* times_out)
 {
+    using R = typename P::R;
+    using S = typename P::S;
+    (void)sizeof(R);
+    (void)sizeof(S);
     void* buf = scratch;
     bool own = false;
     if (!buf) {
-        SFD_HIP(hipMalloc(&buf, (size_t)c->nsite * sizeof(cx<R>)));
+        SFD_HIP(hipMalloc(&buf, (size_t)c->nsite * sizeof(cx<S>)));
         own = true;
     }
     int rc = SFD_OK;
@@ -517,21 +558,21 @@ int autotune_impl(sfd_ctx* c, void* scratch, float* times_out)
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
         rc = fail(SFD_ERR_HIP, "hipEventCreate failed");
     }
-    if (rc == SFD_OK && hipMemsetAsync(buf, 0, (size_t)c->nsite * sizeof(cx<R>), c->stream) != hipSuccess)
+    if (rc == SFD_OK && hipMemsetAsync(buf, 0, (size_t)c->nsite * sizeof(cx<S>), c->stream) != hipSuccess)
         rc = fail(SFD_ERR_HIP, "hipMemsetAsync failed");
     for (int ax = 0; rc == SFD_OK && ax < c->ndim; ++ax) {
-        const int nv = n_variants<R>(c->n[ax], ax == 0);
+        const int nv = n_variants<P>(c->n[ax], ax == 0);
         int best = 0;
         float best_ms = 1e30f;
         for (int v = 0; rc == SFD_OK && v < nv; ++v) {
-            if ((rc = set_variant<R>(c, ax, v)) != SFD_OK) break;
+            if ((rc = set_variant<P>(c, ax, v)) != SFD_OK) break;
             float vbest = 1e30f;
             for (int w = 0; rc == SFD_OK && w < 2; ++w)
-                rc = round_trip<R>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, 0, 0.0, 0.0);
+                rc = round_trip<P>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, 0, 0.0, 0.0);
             for (int rep = 0; rc == SFD_OK && rep < 3; ++rep) {
                 (void)hipEventRecord(e0, c->stream);
                 for (int k = 0; rc == SFD_OK && k < 8; ++k)
-                    rc = round_trip<R>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, 0, 0.0, 0.0);
+                    rc = round_trip<P>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, 0, 0.0, 0.0);
                 (void)hipEventRecord(e1, c->stream);
                 if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(SFD_ERR_HIP, "event sync failed"); break; }
                 float ms = 0.f;
@@ -541,7 +582,7 @@ int autotune_impl(sfd_ctx* c, void* scratch, float* times_out)
             if (times_out) times_out[ax * 8 + (v < 8 ? v : 7)] = vbest;
             if (vbest < best_ms) { best_ms = vbest; best = v; }
         }
-        if (rc == SFD_OK) rc = set_variant<R>(c, ax, best);
+        if (rc == SFD_OK) rc = set_variant<P>(c, ax, best);
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -553,12 +594,16 @@ int autotune_impl(sfd_ctx* c, void* scratch, float* times_out)
     return rc;
 }
 
-template <typename R>
+template <typename P>
 int init_tables(sfd_ctx* c)
 {
+    using R = typename P::R;
+    using S = typename P::S;
+    (void)sizeof(R);
+    (void)sizeof(S);
     for (int ax = 0; ax < c->ndim; ++ax) {
         const int n = c->n[ax];
-        SFD_TRY(set_variant<R>(c, ax, 0));
+        SFD_TRY(set_variant<P>(c, ax, 0));
         std::vector<double> xi(n), co(n);
         for (int i = 0; i < n; ++i) {
             xi[i] = fft_freq(n, c->d[ax], i);
@@ -590,6 +635,11 @@ void release(sfd_ctx* c)
 
 }  // namespace
 
+// Call FN<precision traits>(args...) for the context's precision.
+#define SFD_BY_PREC(ctx, FN, ...)                                         \
+    ((ctx)->prec == SFD_F32 ? FN<PF32>(__VA_ARGS__)                       \
+                            : (ctx)->prec == SFD_F64 ? FN<PF64>(__VA_ARGS__) : FN<PF32X>(__VA_ARGS__))
+
 // ===================================================================== C ABI
 extern "C" {
 
@@ -604,8 +654,8 @@ int sfd_ctx_create(sfd_ctx** out, int ndim, const int64_t* n, const double* box_
     if (!out || !n || !box_min || !box_max) return fail(SFD_ERR_NULL, "sfd_ctx_create: null argument");
     *out = nullptr;
     if (ndim < 2 || ndim > 3) return fail(SFD_ERR_SHAPE, "ndim must be 2 or 3 (got %d)", ndim);
-    if (precision != SFD_F32 && precision != SFD_F64)
-        return fail(SFD_ERR_ARG, "precision must be SFD_F32 or SFD_F64 (got %d)", precision);
+    if (precision != SFD_F32 && precision != SFD_F64 && precision != SFD_F32X)
+        return fail(SFD_ERR_ARG, "precision must be SFD_F32, SFD_F64 or SFD_F32X (got %d)", precision);
     for (int a = 0; a < ndim; ++a) {
         const int64_t m = n[a];
         if (m < 2 || m > 1024 || (m & (m - 1)))
@@ -633,7 +683,7 @@ int sfd_ctx_create(sfd_ctx** out, int ndim, const int64_t* n, const double* box_
     if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
         c->ncu < 1)
         c->ncu = 1;
-    int rc = precision == SFD_F32 ? init_tables<float>(c) : init_tables<double>(c);
+    int rc = SFD_BY_PREC(c, init_tables, c);
     if (rc != SFD_OK) {
         release(c);
         delete c;
@@ -670,8 +720,7 @@ int sfd_split_step(sfd_ctx* ctx, void* psi, const void* V, double coeffD, double
     if (nsteps < 0) return fail(SFD_ERR_ARG, "nsteps must be >= 0");
     if (!std::isfinite(coeffD) || !std::isfinite(beta) || !std::isfinite(Omega) || !std::isfinite(dt))
         return fail(SFD_ERR_ARG, "non-finite scalar argument");
-    if (ctx->prec == SFD_F32) return split_impl<float>(ctx, psi, V, coeffD, beta, Omega, dt, nsteps, order, mode);
-    return split_impl<double>(ctx, psi, V, coeffD, beta, Omega, dt, nsteps, order, mode);
+    return SFD_BY_PREC(ctx, split_impl, ctx, psi, V, coeffD, beta, Omega, dt, nsteps, order, mode);
 }
 
 int sfd_lap_rot_axis(sfd_ctx* ctx, void* psi, int axis, double coeffD, double Omega, double tau, int mode)
@@ -683,8 +732,7 @@ int sfd_lap_rot_axis(sfd_ctx* ctx, void* psi, int axis, double coeffD, double Om
     if (!std::isfinite(coeffD) || !std::isfinite(Omega) || !std::isfinite(tau))
         return fail(SFD_ERR_ARG, "non-finite scalar argument");
     const int imag = mode == SFD_IMAG_TIME;
-    if (ctx->prec == SFD_F32) return round_trip<float>(ctx, psi, nullptr, axis, tau, coeffD, Omega, imag, 0, 0.0, 0.0);
-    return round_trip<double>(ctx, psi, nullptr, axis, tau, coeffD, Omega, imag, 0, 0.0, 0.0);
+    return SFD_BY_PREC(ctx, round_trip, ctx, psi, nullptr, axis, tau, coeffD, Omega, imag, 0, 0.0, 0.0);
 }
 
 int sfd_lap_rot(sfd_ctx* ctx, void* psi, double coeffD, double Omega, double tau, int mode)
@@ -703,10 +751,14 @@ int sfd_solve_nl(sfd_ctx* ctx, void* psi, const void* V, double beta, double dt,
     const long long nb = grid_blocks(ctx->nsite);
     const int imag = mode == SFD_IMAG_TIME;
     if (ctx->prec == SFD_F32)
-        hipLaunchKernelGGL(k_nl<float>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, static_cast<cx<float>*>(psi),
-                           static_cast<const float*>(V), ctx->nsite, (float)beta, (float)dt, imag);
+        hipLaunchKernelGGL((k_nl<float, float>), dim3((unsigned)nb), dim3(256), 0, ctx->stream,
+                           static_cast<cx<float>*>(psi), static_cast<const float*>(V), ctx->nsite, (float)beta,
+                           (float)dt, imag);
+    else if (ctx->prec == SFD_F32X)
+        hipLaunchKernelGGL((k_nl<float, double>), dim3((unsigned)nb), dim3(256), 0, ctx->stream,
+                           static_cast<cx<float>*>(psi), static_cast<const float*>(V), ctx->nsite, beta, dt, imag);
     else
-        hipLaunchKernelGGL(k_nl<double>, dim3((unsigned)nb), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL((k_nl<double, double>), dim3((unsigned)nb), dim3(256), 0, ctx->stream,
                            static_cast<cx<double>*>(psi), static_cast<const double*>(V), ctx->nsite, beta, dt, imag);
     SFD_HIP(hipGetLastError());
     return SFD_OK;
@@ -716,8 +768,7 @@ int sfd_norm2(sfd_ctx* ctx, const void* psi, double* out_sum)
 {
     SFD_TRY(check_ctx(ctx));
     if (!psi || !out_sum) return fail(SFD_ERR_NULL, "sfd_norm2: null argument");
-    int rc = ctx->prec == SFD_F32 ? norm2_impl<float>(ctx, psi, ctx->scal + 1)
-                                  : norm2_impl<double>(ctx, psi, ctx->scal + 1);
+    int rc = SFD_BY_PREC(ctx, norm2_impl, ctx, psi, ctx->scal + 1);
     if (rc != SFD_OK) return rc;
     SFD_HIP(hipMemcpyAsync(out_sum, ctx->scal + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     SFD_HIP(hipStreamSynchronize(ctx->stream));
@@ -733,7 +784,7 @@ int sfd_normalize(sfd_ctx* ctx, void* psi, double* out_norm)
     if (out_norm) *out_norm = nrm;
     if (!(nrm > 0.0) || !std::isfinite(nrm)) return fail(SFD_ERR_ARG, "normalize: norm is %g", nrm);
     const long long nb = grid_blocks(ctx->nsite);
-    if (ctx->prec == SFD_F32)
+    if (ctx->prec != SFD_F64)
         hipLaunchKernelGGL(k_scale<float>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, static_cast<cx<float>*>(psi),
                            ctx->nsite, nrm, (const double*)nullptr, 1);
     else
@@ -748,8 +799,7 @@ int sfd_energy(sfd_ctx* ctx, const void* psi, const void* V, double coeffD, doub
 {
     SFD_TRY(check_ctx(ctx));
     if (!psi || !out5) return fail(SFD_ERR_NULL, "sfd_energy: null argument");
-    if (ctx->prec == SFD_F32) return energy_impl<float>(ctx, psi, V, coeffD, beta, Omega, out5);
-    return energy_impl<double>(ctx, psi, V, coeffD, beta, Omega, out5);
+    return SFD_BY_PREC(ctx, energy_impl, ctx, psi, V, coeffD, beta, Omega, out5);
 }
 
 int sfd_fft_axis(sfd_ctx* ctx, const void* in, void* out, int axis, int direction)
@@ -758,8 +808,7 @@ int sfd_fft_axis(sfd_ctx* ctx, const void* in, void* out, int axis, int directio
     if (!in || !out) return fail(SFD_ERR_NULL, "sfd_fft_axis: null argument");
     if (axis < 0 || axis >= ctx->ndim) return fail(SFD_ERR_ARG, "axis %d out of range", axis);
     if (direction != SFD_FORWARD && direction != SFD_BACKWARD) return fail(SFD_ERR_ARG, "direction must be -1 or 1");
-    if (ctx->prec == SFD_F32) return fft_impl<float>(ctx, in, out, axis, direction);
-    return fft_impl<double>(ctx, in, out, axis, direction);
+    return SFD_BY_PREC(ctx, fft_impl, ctx, in, out, axis, direction);
 }
 
 int sfd_autotune(sfd_ctx* ctx, void* scratch, float* times_ms)
@@ -767,8 +816,7 @@ int sfd_autotune(sfd_ctx* ctx, void* scratch, float* times_ms)
     SFD_TRY(check_ctx(ctx));
     if (times_ms)
         for (int i = 0; i < 24; ++i) times_ms[i] = -1.f;
-    if (ctx->prec == SFD_F32) return autotune_impl<float>(ctx, scratch, times_ms);
-    return autotune_impl<double>(ctx, scratch, times_ms);
+    return SFD_BY_PREC(ctx, autotune_impl, ctx, scratch, times_ms);
 }
 
 int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* shape5)
@@ -776,8 +824,7 @@ int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* sha
     if (!ctx || !index || !count) return fail(SFD_ERR_NULL, "sfd_ctx_get_variant: null argument");
     if (axis < 0 || axis >= ctx->ndim) return fail(SFD_ERR_ARG, "axis %d out of range", axis);
     *index = ctx->var[axis];
-    *count = ctx->prec == SFD_F32 ? n_variants<float>(ctx->n[axis], axis == 0)
-                                  : n_variants<double>(ctx->n[axis], axis == 0);
+    *count = SFD_BY_PREC(ctx, n_variants, ctx->n[axis], axis == 0);
     if (shape5) {
         shape5[0] = ctx->vi[axis].n1;
         shape5[1] = ctx->vi[axis].n2;
@@ -792,8 +839,7 @@ int sfd_ctx_set_variant(sfd_ctx* ctx, int axis, int index)
 {
     SFD_TRY(check_ctx(ctx));
     if (axis < 0 || axis >= ctx->ndim) return fail(SFD_ERR_ARG, "axis %d out of range", axis);
-    if (ctx->prec == SFD_F32) return set_variant<float>(ctx, axis, index);
-    return set_variant<double>(ctx, axis, index);
+    return SFD_BY_PREC(ctx, set_variant, ctx, axis, index);
 }
 
 int sfd_alloc(sfd_ctx* ctx, size_t bytes, void** out)

@@ -180,34 +180,41 @@ __device__ __forceinline__ const T& lane_ref(const T* base, long long off, unsig
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base + off) + (size_t)lane * sizeof(T));
 }
 
-template <typename R, int N1>
-__device__ __forceinline__ void load_line(cx<R> (&v)[N1], const cx<R>* psi, const TileGeom& g)
+// Storage precision S (HBM) and compute precision R (registers, LDS) may
+// differ: complex FP32 fields can be transformed in FP64 arithmetic.
+template <typename R, typename S>
+__device__ __forceinline__ cx<R> widen(cx<S> x) { return {R(x.re), R(x.im)}; }
+template <typename S, typename R>
+__device__ __forceinline__ cx<S> narrow(cx<R> x) { return {S(x.re), S(x.im)}; }
+
+template <typename S, typename R, int N1>
+__device__ __forceinline__ void load_line(cx<R> (&v)[N1], const cx<S>* psi, const TileGeom& g)
 {
-    const cx<R>* b = psi + g.ubase;
+    const cx<S>* b = psi + g.ubase;
 #pragma unroll
-    for (int j = 0; j < N1; ++j) v[j] = lane_ref(b, j * g.jstride, g.lane);
+    for (int j = 0; j < N1; ++j) v[j] = widen<R>(lane_ref(b, j * g.jstride, g.lane));
 }
 
-template <typename R, int N1>
-__device__ __forceinline__ void load_pot(R (&vp)[N1], const R* V, const TileGeom& g)
+template <typename S, typename R, int N1>
+__device__ __forceinline__ void load_pot(R (&vp)[N1], const S* V, const TileGeom& g)
 {
     if (V) {
-        const R* b = V + g.ubase;
+        const S* b = V + g.ubase;
 #pragma unroll
-        for (int j = 0; j < N1; ++j) vp[j] = lane_ref(b, j * g.jstride, g.lane);
+        for (int j = 0; j < N1; ++j) vp[j] = R(lane_ref(b, j * g.jstride, g.lane));
     } else {
 #pragma unroll
         for (int j = 0; j < N1; ++j) vp[j] = R(0);
     }
 }
 
-template <typename R, int N1>
-__device__ __forceinline__ void store_line(const cx<R> (&v)[N1], cx<R>* psi, const TileGeom& g)
+template <typename S, typename R, int N1>
+__device__ __forceinline__ void store_line(const cx<R> (&v)[N1], cx<S>* psi, const TileGeom& g)
 {
     if (g.active) {
-        cx<R>* b = psi + g.ubase;
+        cx<S>* b = psi + g.ubase;
 #pragma unroll
-        for (int j = 0; j < N1; ++j) lane_ref(b, j * g.jstride, g.lane) = v[j];
+        for (int j = 0; j < N1; ++j) lane_ref(b, j * g.jstride, g.lane) = narrow<S>(v[j]);
     }
 }
 
@@ -299,7 +306,7 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
 }
 
 // One launch over all tiles of an axis, one tile per workgroup; every mode.
-template <typename R, int N1, int N2, int TS, int G, int MODE>
+template <typename S, typename R, int N1, int N2, int TS, int G, int MODE>
 __global__ __launch_bounds__(TS * G * N2) void k_sweep(const SweepArgs a)
 {
     using Sh = SweepShape<R, N1, N2, TS, G>;
@@ -318,50 +325,52 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep(const SweepArgs a)
     const int flags = a.flags;
     const cx<R>* __restrict__ twA = static_cast<const cx<R>*>(a.tw) + t;
     const cx<R>* __restrict__ twB = static_cast<const cx<R>*>(a.tw) + N + t;
-    cx<R>* __restrict__ psi = static_cast<cx<R>*>(a.psi);
+    cx<S>* __restrict__ psi = static_cast<cx<S>*>(a.psi);
     const long long nblocks = (long long)gridDim.x * gridDim.y;
 
     cx<R> v[N1];
     R vp[N1];
 
     if constexpr (MODE == kRoundTrip) {
-        load_line<R, N1>(v, psi, gm);
-        if (flags & kFlagNL) load_pot<R, N1>(vp, static_cast<const R*>(a.V), gm);
+        load_line<S, R, N1>(v, psi, gm);
+        if (flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
         double nacc = 0.0;
         round_trip_tile<R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
         if (flags & kFlagNorm) {
             double acc1[1] = {nacc};
             block_partials<R, Sh::NT, 1>(acc1, a.partial, nblocks, tile);
         }
-        store_line<R, N1>(v, psi, gm);
+        store_line<S, R, N1>(v, psi, gm);
     } else if constexpr (MODE == kForward) {
         cx<R> w[RB][N2];
-        load_line<R, N1>(v, psi, gm);
+        load_line<S, R, N1>(v, psi, gm);
         forward_ab<R, N1, N2, TS, G>(v, w, lds, twA, c, g, t);
         if (gm.active) {
-            cx<R>* __restrict__ ob = static_cast<cx<R>*>(a.out) + gm.ubase;
+            cx<S>* __restrict__ ob = static_cast<cx<S>*>(a.out) + gm.ubase;
 #pragma unroll
             for (int r = 0; r < RB; ++r)
 #pragma unroll
-                for (int k2 = 0; k2 < N2; ++k2) lane_ref(ob, (N2 * r + N1 * k2) * gm.kstride, gm.lane) = w[r][k2];
+                for (int k2 = 0; k2 < N2; ++k2)
+                    lane_ref(ob, (N2 * r + N1 * k2) * gm.kstride, gm.lane) = narrow<S>(w[r][k2]);
         }
     } else if constexpr (MODE == kInverse) {
         cx<R> w[RB][N2];
-        const cx<R>* ib = psi + gm.ubase;
+        const cx<S>* ib = psi + gm.ubase;
 #pragma unroll
         for (int r = 0; r < RB; ++r)
 #pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) w[r][k2] = lane_ref(ib, (N2 * r + N1 * k2) * gm.kstride, gm.lane);
+            for (int k2 = 0; k2 < N2; ++k2)
+                w[r][k2] = widen<R>(lane_ref(ib, (N2 * r + N1 * k2) * gm.kstride, gm.lane));
         inverse_ba<R, N1, N2, TS, G>(w, v, lds, twB, c, g, t);
         const R inv = R(1) / R(N);
 #pragma unroll
         for (int j = 0; j < N1; ++j) v[j] = cscale(v[j], inv);
-        store_line<R, N1>(v, static_cast<cx<R>*>(a.out), gm);
+        store_line<S, R, N1>(v, static_cast<cx<S>*>(a.out), gm);
     } else {  // kEnergy
         double acc[5] = {0, 0, 0, 0, 0};
-        load_line<R, N1>(v, psi, gm);
+        load_line<S, R, N1>(v, psi, gm);
         if (flags & kFlagPointwise) {
-            load_pot<R, N1>(vp, static_cast<const R*>(a.V), gm);
+            load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
 #pragma unroll
             for (int j = 0; j < N1; ++j) {
                 const double re = v[j].re, im = v[j].im;
@@ -397,7 +406,7 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep(const SweepArgs a)
 // blockIdx.x, blockIdx.x + gridDim.x, ... and issues the loads of its next
 // tile before transforming the current one, so HBM traffic overlaps the
 // FFT arithmetic within every wavefront.
-template <typename R, int N1, int N2, int TS, int G>
+template <typename S, typename R, int N1, int N2, int TS, int G>
 __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
 {
     using Sh = SweepShape<R, N1, N2, TS, G>;
@@ -414,8 +423,8 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
     const bool nl = flags & kFlagNL;
     const cx<R>* __restrict__ twA = static_cast<const cx<R>*>(a.tw) + t;
     const cx<R>* __restrict__ twB = static_cast<const cx<R>*>(a.tw) + N + t;
-    cx<R>* __restrict__ psi = static_cast<cx<R>*>(a.psi);
-    const R* __restrict__ V = static_cast<const R*>(a.V);
+    cx<S>* __restrict__ psi = static_cast<cx<S>*>(a.psi);
+    const S* __restrict__ V = static_cast<const S*>(a.V);
 
     long long tile = blockIdx.x;
     double nacc = 0.0;
@@ -423,8 +432,8 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
     R vp[N1];
     TileGeom gm = tile_geom<N, N2, TS, NL>(a, tile, L, t);
     if (tile < a.ntiles) {
-        load_line<R, N1>(v, psi, gm);
-        if (nl) load_pot<R, N1>(vp, V, gm);
+        load_line<S, R, N1>(v, psi, gm);
+        if (nl) load_pot<S, R, N1>(vp, V, gm);
     }
     while (tile < a.ntiles) {
         const long long next = tile + gridDim.x;
@@ -434,11 +443,11 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
         R vpn[N1];
         if (more) {
             gn = tile_geom<N, N2, TS, NL>(a, next, L, t);
-            load_line<R, N1>(vn, psi, gn);
-            if (nl) load_pot<R, N1>(vpn, V, gn);
+            load_line<S, R, N1>(vn, psi, gn);
+            if (nl) load_pot<S, R, N1>(vpn, V, gn);
         }
         round_trip_tile<R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
-        store_line<R, N1>(v, psi, gm);
+        store_line<S, R, N1>(v, psi, gm);
         if (!more) break;
         __syncthreads();  // every lane is done with the exchange buffer
 #pragma unroll

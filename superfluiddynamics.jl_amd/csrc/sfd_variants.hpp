@@ -24,10 +24,24 @@ struct VL {
     static constexpr int size = sizeof...(Vs);
 };
 
+// Precision of a context: storage scalar S (HBM) and compute scalar R.
+struct PF32 {
+    using S = float;
+    using R = float;
+};
+struct PF64 {
+    using S = double;
+    using R = double;
+};
+struct PF32X {  // complex FP32 fields, FP64 arithmetic
+    using S = float;
+    using R = double;
+};
+
 // Per (precision, N): contiguous (x axis) and strided (y/z axes) variants.
 // Every variant keeps its exchange buffer within 64 KiB of LDS and at least
 // one full wavefront per workgroup.
-template <typename R, int N>
+template <typename P, int N>
 struct Variants;
 
 // the lists are passed parenthesised, so that their commas survive the macro
@@ -46,69 +60,76 @@ struct Unparen<void(T)> {
     };
 
 // ---------------------------------------------------------------- FP32
-SFD_VARIANTS(float, 2, (VL<V<2, 1, 1, 256>>), (VL<V<2, 1, 16, 4>>))
-SFD_VARIANTS(float, 4, (VL<V<2, 2, 1, 128>>), (VL<V<2, 2, 16, 2>>))
-SFD_VARIANTS(float, 8, (VL<V<4, 2, 1, 128>>), (VL<V<4, 2, 16, 2>>))
-SFD_VARIANTS(float, 16, (VL<V<4, 4, 1, 64>>), (VL<V<4, 4, 16, 1>>))
-SFD_VARIANTS(float, 32, (VL<V<8, 4, 1, 64>>), (VL<V<8, 4, 16, 1>>))
-SFD_VARIANTS(float, 64, (VL<V<8, 8, 1, 32>, V<16, 4, 1, 64>>), (VL<V<8, 8, 16, 1>, V<8, 8, 32, 1>>))
-SFD_VARIANTS(float, 128, (VL<V<16, 8, 1, 32>, V<16, 8, 1, 16>, V<32, 4, 1, 32>, V<16, 8, 1, 16, 1>>),
+SFD_VARIANTS(PF32, 2, (VL<V<2, 1, 1, 256>>), (VL<V<2, 1, 16, 4>>))
+SFD_VARIANTS(PF32, 4, (VL<V<2, 2, 1, 128>>), (VL<V<2, 2, 16, 2>>))
+SFD_VARIANTS(PF32, 8, (VL<V<4, 2, 1, 128>>), (VL<V<4, 2, 16, 2>>))
+SFD_VARIANTS(PF32, 16, (VL<V<4, 4, 1, 64>>), (VL<V<4, 4, 16, 1>>))
+SFD_VARIANTS(PF32, 32, (VL<V<8, 4, 1, 64>>), (VL<V<8, 4, 16, 1>>))
+SFD_VARIANTS(PF32, 64, (VL<V<8, 8, 1, 32>, V<16, 4, 1, 64>>), (VL<V<8, 8, 16, 1>, V<8, 8, 32, 1>>))
+SFD_VARIANTS(PF32, 128, (VL<V<16, 8, 1, 32>, V<16, 8, 1, 16>, V<32, 4, 1, 32>, V<16, 8, 1, 16, 1>>),
              (VL<V<16, 8, 16, 1>, V<16, 8, 32, 1>, V<32, 4, 32, 1>, V<16, 8, 16, 1, 1>>))
-SFD_VARIANTS(float, 256,
+SFD_VARIANTS(PF32, 256,
              (VL<V<16, 16, 1, 16>, V<16, 16, 1, 8>, V<32, 8, 1, 16>, V<32, 8, 1, 8>, V<16, 16, 1, 16, 1>,
                  V<16, 16, 1, 8, 1>>),
              (VL<V<16, 16, 16, 1>, V<16, 16, 32, 1>, V<32, 8, 16, 1>, V<32, 8, 32, 1>, V<16, 16, 16, 1, 1>,
                  V<16, 16, 8, 1, 1>>))
-SFD_VARIANTS(float, 512, (VL<V<32, 16, 1, 8>, V<32, 16, 1, 4>, V<32, 16, 1, 4, 1>>),
+SFD_VARIANTS(PF32, 512, (VL<V<32, 16, 1, 8>, V<32, 16, 1, 4>, V<32, 16, 1, 4, 1>>),
              (VL<V<32, 16, 16, 1>, V<32, 16, 8, 1>, V<32, 16, 8, 1, 1>>))
-SFD_VARIANTS(float, 1024, (VL<V<32, 32, 1, 4>, V<32, 32, 1, 2>>), (VL<V<32, 32, 8, 1>, V<32, 32, 4, 1>>))
+SFD_VARIANTS(PF32, 1024, (VL<V<32, 32, 1, 4>, V<32, 32, 1, 2>>), (VL<V<32, 32, 8, 1>, V<32, 32, 4, 1>>))
 
 // ---------------------------------------------------------------- FP64
-SFD_VARIANTS(double, 2, (VL<V<2, 1, 1, 256>>), (VL<V<2, 1, 16, 4>>))
-SFD_VARIANTS(double, 4, (VL<V<2, 2, 1, 128>>), (VL<V<2, 2, 16, 2>>))
-SFD_VARIANTS(double, 8, (VL<V<4, 2, 1, 128>>), (VL<V<4, 2, 16, 2>>))
-SFD_VARIANTS(double, 16, (VL<V<4, 4, 1, 64>>), (VL<V<4, 4, 16, 1>>))
-SFD_VARIANTS(double, 32, (VL<V<8, 4, 1, 64>>), (VL<V<8, 4, 16, 1>>))
-SFD_VARIANTS(double, 64, (VL<V<8, 8, 1, 32>, V<16, 4, 1, 32>>), (VL<V<8, 8, 16, 1>, V<8, 8, 32, 1>>))
-SFD_VARIANTS(double, 128, (VL<V<16, 8, 1, 16>, V<16, 8, 1, 8>, V<32, 4, 1, 16>>),
+SFD_VARIANTS(PF64, 2, (VL<V<2, 1, 1, 256>>), (VL<V<2, 1, 16, 4>>))
+SFD_VARIANTS(PF64, 4, (VL<V<2, 2, 1, 128>>), (VL<V<2, 2, 16, 2>>))
+SFD_VARIANTS(PF64, 8, (VL<V<4, 2, 1, 128>>), (VL<V<4, 2, 16, 2>>))
+SFD_VARIANTS(PF64, 16, (VL<V<4, 4, 1, 64>>), (VL<V<4, 4, 16, 1>>))
+SFD_VARIANTS(PF64, 32, (VL<V<8, 4, 1, 64>>), (VL<V<8, 4, 16, 1>>))
+SFD_VARIANTS(PF64, 64, (VL<V<8, 8, 1, 32>, V<16, 4, 1, 32>>), (VL<V<8, 8, 16, 1>, V<8, 8, 32, 1>>))
+SFD_VARIANTS(PF64, 128, (VL<V<16, 8, 1, 16>, V<16, 8, 1, 8>, V<32, 4, 1, 16>>),
              (VL<V<16, 8, 16, 1>, V<16, 8, 32, 1>, V<32, 4, 16, 1>>))
-SFD_VARIANTS(double, 256, (VL<V<16, 16, 1, 8>, V<16, 16, 1, 4>, V<16, 16, 1, 8, 1>, V<16, 16, 1, 4, 1>>),
+SFD_VARIANTS(PF64, 256, (VL<V<16, 16, 1, 8>, V<16, 16, 1, 4>, V<16, 16, 1, 8, 1>, V<16, 16, 1, 4, 1>>),
              (VL<V<16, 16, 16, 1>, V<16, 16, 8, 1>, V<16, 16, 16, 1, 1>, V<16, 16, 8, 1, 1>>))
-SFD_VARIANTS(double, 512, (VL<V<32, 16, 1, 4>>), (VL<V<32, 16, 8, 1>, V<32, 16, 4, 1>>))
-SFD_VARIANTS(double, 1024, (VL<V<32, 32, 1, 2>>), (VL<V<32, 32, 4, 1>, V<32, 32, 2, 1>>))
+SFD_VARIANTS(PF64, 512, (VL<V<32, 16, 1, 4>>), (VL<V<32, 16, 8, 1>, V<32, 16, 4, 1>>))
+SFD_VARIANTS(PF64, 1024, (VL<V<32, 32, 1, 2>>), (VL<V<32, 32, 4, 1>, V<32, 32, 2, 1>>))
 
 #undef SFD_VARIANTS
 
-template <typename R, int MODE, typename L>
+// FP32 storage with FP64 arithmetic has the registers and LDS of FP64.
+template <int N>
+struct Variants<PF32X, N> : Variants<PF64, N> {
+};
+
+template <typename P, int MODE, typename L>
 struct Dispatch;
 
-template <typename R, int MODE>
-struct Dispatch<R, MODE, VL<>> {
+template <typename P, int MODE>
+struct Dispatch<P, MODE, VL<>> {
     static bool info(int, VariantInfo*) { return false; }
     static bool launch(int, dim3, dim3, hipStream_t, const SweepArgs&) { return false; }
     static int occupancy(int) { return 0; }
 };
 
-template <typename R, int MODE, typename V0, typename... Vs>
-struct Dispatch<R, MODE, VL<V0, Vs...>> {
+template <typename P, int MODE, typename V0, typename... Vs>
+struct Dispatch<P, MODE, VL<V0, Vs...>> {
+    using S = typename P::S;
+    using R = typename P::R;
     static bool info(int idx, VariantInfo* out)
     {
         if (idx == 0) {
             *out = {V0::N1, V0::N2, V0::TS, V0::G, V0::P};
             return true;
         }
-        return Dispatch<R, MODE, VL<Vs...>>::info(idx - 1, out);
+        return Dispatch<P, MODE, VL<Vs...>>::info(idx - 1, out);
     }
     static bool launch(int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a)
     {
         if (idx == 0) {
             if constexpr (V0::P && MODE == kRoundTrip)
-                hipLaunchKernelGGL((k_sweep_pipe<R, V0::N1, V0::N2, V0::TS, V0::G>), grid, block, 0, s, a);
+                hipLaunchKernelGGL((k_sweep_pipe<S, R, V0::N1, V0::N2, V0::TS, V0::G>), grid, block, 0, s, a);
             else
-                hipLaunchKernelGGL((k_sweep<R, V0::N1, V0::N2, V0::TS, V0::G, MODE>), grid, block, 0, s, a);
+                hipLaunchKernelGGL((k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE>), grid, block, 0, s, a);
             return true;
         }
-        return Dispatch<R, MODE, VL<Vs...>>::launch(idx - 1, grid, block, s, a);
+        return Dispatch<P, MODE, VL<Vs...>>::launch(idx - 1, grid, block, s, a);
     }
     // resident workgroups per CU of the persistent round-trip kernel
     static int occupancy(int idx)
@@ -117,82 +138,82 @@ struct Dispatch<R, MODE, VL<V0, Vs...>> {
             if constexpr (V0::P) {
                 int nb = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &nb, reinterpret_cast<const void*>(&k_sweep_pipe<R, V0::N1, V0::N2, V0::TS, V0::G>),
+                        &nb, reinterpret_cast<const void*>(&k_sweep_pipe<S, R, V0::N1, V0::N2, V0::TS, V0::G>),
                         V0::TS * V0::G * V0::N2, 0) != hipSuccess)
                     return 1;
                 return nb > 0 ? nb : 1;
             }
             return 0;
         }
-        return Dispatch<R, MODE, VL<Vs...>>::occupancy(idx - 1);
+        return Dispatch<P, MODE, VL<Vs...>>::occupancy(idx - 1);
     }
 };
 
 // Runtime lookup by line length.
-template <typename R, int N>
+template <typename P, int N>
 int variant_count(bool contig)
 {
-    return contig ? Variants<R, N>::contig::size : Variants<R, N>::strided::size;
+    return contig ? Variants<P, N>::contig::size : Variants<P, N>::strided::size;
 }
 
-template <typename R, int MODE, int N>
+template <typename P, int MODE, int N>
 bool launch_variant_n(bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a)
 {
-    if (contig) return Dispatch<R, MODE, typename Variants<R, N>::contig>::launch(idx, grid, block, s, a);
-    return Dispatch<R, MODE, typename Variants<R, N>::strided>::launch(idx, grid, block, s, a);
+    if (contig) return Dispatch<P, MODE, typename Variants<P, N>::contig>::launch(idx, grid, block, s, a);
+    return Dispatch<P, MODE, typename Variants<P, N>::strided>::launch(idx, grid, block, s, a);
 }
 
-template <typename R, int N>
+template <typename P, int N>
 bool info_variant_n(bool contig, int idx, VariantInfo* out)
 {
-    if (contig) return Dispatch<R, 0, typename Variants<R, N>::contig>::info(idx, out);
-    return Dispatch<R, 0, typename Variants<R, N>::strided>::info(idx, out);
+    if (contig) return Dispatch<P, 0, typename Variants<P, N>::contig>::info(idx, out);
+    return Dispatch<P, 0, typename Variants<P, N>::strided>::info(idx, out);
 }
 
 #define SFD_FOR_EACH_N(M) M(2) M(4) M(8) M(16) M(32) M(64) M(128) M(256) M(512) M(1024)
 
-template <typename R>
+template <typename P>
 int n_variants(int n, bool contig)
 {
 #define SFD_CASE(N_) \
-    if (n == N_) return variant_count<R, N_>(contig);
+    if (n == N_) return variant_count<P, N_>(contig);
     SFD_FOR_EACH_N(SFD_CASE)
 #undef SFD_CASE
     return 0;
 }
 
-template <typename R, int N>
+template <typename P, int N>
 int occupancy_variant_n(bool contig, int idx)
 {
-    if (contig) return Dispatch<R, 0, typename Variants<R, N>::contig>::occupancy(idx);
-    return Dispatch<R, 0, typename Variants<R, N>::strided>::occupancy(idx);
+    if (contig) return Dispatch<P, 0, typename Variants<P, N>::contig>::occupancy(idx);
+    return Dispatch<P, 0, typename Variants<P, N>::strided>::occupancy(idx);
 }
 
-template <typename R>
+template <typename P>
 int variant_occupancy(int n, bool contig, int idx)
 {
 #define SFD_CASE(N_) \
-    if (n == N_) return occupancy_variant_n<R, N_>(contig, idx);
+    if (n == N_) return occupancy_variant_n<P, N_>(contig, idx);
     SFD_FOR_EACH_N(SFD_CASE)
 #undef SFD_CASE
     return 0;
 }
 
-template <typename R>
+template <typename P>
 bool variant_info(int n, bool contig, int idx, VariantInfo* out)
 {
 #define SFD_CASE(N_) \
-    if (n == N_) return info_variant_n<R, N_>(contig, idx, out);
+    if (n == N_) return info_variant_n<P, N_>(contig, idx, out);
     SFD_FOR_EACH_N(SFD_CASE)
 #undef SFD_CASE
     return false;
 }
 
-template <typename R, int MODE>
+template <typename P, int MODE>
 bool launch_variant(int n, bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a)
 {
 #define SFD_CASE(N_) \
-    if (n == N_) return launch_variant_n<R, MODE, N_>(contig, idx, grid, block, s, a);
+    if (n == N_) return launch_variant_n<P, MODE, N_>(contig, idx, grid, block, s, a);
     SFD_FOR_EACH_N(SFD_CASE)
 #undef SFD_CASE
     return false;

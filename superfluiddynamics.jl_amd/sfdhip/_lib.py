@@ -22,6 +22,7 @@ SFD_ERR_HIP = -5
 
 SFD_F32 = 1
 SFD_F64 = 2
+SFD_F32X = 3
 SFD_REAL_TIME = 0
 SFD_IMAG_TIME = 1
 SFD_FORWARD = -1

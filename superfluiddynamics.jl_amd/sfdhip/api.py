@@ -74,7 +74,7 @@ def fftfreq(n: int, d: float) -> np.ndarray:
 class Grid:
     """Grid{N,FT,A} (Grids.jl:24-129).  ``size`` and ``bounds`` are x-first."""
 
-    def __init__(self, size, bounds, FT=np.float64, array_type=HipArray):
+    def __init__(self, size, bounds, FT=np.float64, array_type=HipArray, compute=None):
         size = tuple(int(s) for s in size)
         if len(size) not in (2, 3) or len(bounds) != len(size):
             raise L.ArgError(L.SFD_ERR_ARG, "Grid: 2D or 3D size with one (min, max) per axis")
@@ -92,6 +92,9 @@ class Grid:
         self.FT = np.dtype(FT).type
         if self.FT not in (np.float32, np.float64):
             raise L.ArgError(L.SFD_ERR_ARG, "Grid: FT must be Float32 or Float64")
+        self.compute = np.dtype(compute).type if compute is not None else self.FT
+        if self.compute not in (np.float32, np.float64) or (self.FT is np.float64 and self.compute is np.float32):
+            raise L.ArgError(L.SFD_ERR_ARG, "Grid: compute must be FT or Float64 (FP32 fields with FP64 arithmetic)")
         if array_type is not HipArray:
             raise L.ArgError(L.SFD_ERR_ARG, "Grid: this backend provides array_type=HipArray only")
         self.array_type = array_type
@@ -132,7 +135,10 @@ class _Ctx:
         n = (ctypes.c_int64 * 3)(*(list(grid.n) + [1] * (3 - nd)))
         lo = (ctypes.c_double * 3)(*(list(grid.min) + [0.0] * (3 - nd)))
         hi = (ctypes.c_double * 3)(*(list(grid.max) + [1.0] * (3 - nd)))
-        prec = L.SFD_F32 if grid.FT is np.float32 else L.SFD_F64
+        if grid.FT is np.float64:
+            prec = L.SFD_F64
+        else:
+            prec = L.SFD_F32X if grid.compute is np.float64 else L.SFD_F32
         h = ctypes.c_void_p()
         stream = torch.cuda.current_stream().cuda_stream
         L.check(lib.sfd_ctx_create(ctypes.byref(h), nd, n, lo, hi, prec, ctypes.c_void_p(stream)))

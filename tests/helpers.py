@@ -58,10 +58,17 @@ def random_potential(shape_xfirst, bounds, seed=1, amp=2.0):
     return np.ascontiguousarray(V)
 
 
+PRECS = {"f64": (np.float64, None), "f32": (np.float32, None), "f32x": (np.float32, np.float64)}
+TOLS = {"f64": 1e-12, "f32": 2e-5, "f32x": 1e-6}
+
+
 def gpu_run(sfd, psi0, bounds, *, V=None, FT=np.float64, coeffD=-0.5, beta=1.0, Omega=0.0, dt=1e-3,
-            nsteps=1, order=2, imag=False):
+            nsteps=1, order=2, imag=False, prec=None):
     """Run the HIP split step through the host API on a numpy state."""
-    g = sfd.Grid(tuple(reversed(psi0.shape)), bounds, FT=FT)
+    compute = None
+    if prec is not None:
+        FT, compute = PRECS[prec]
+    g = sfd.Grid(tuple(reversed(psi0.shape)), bounds, FT=FT, compute=compute)
     f = sfd.Field(g, sfd.ComplexField())
     f.set(psi0)
     pot = sfd.PotentialArray(f, V) if V is not None else sfd.PotentialZero(f)

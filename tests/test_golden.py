@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import gpu_run, mesh, rel_l2
+from helpers import TOLS, gpu_run, mesh, rel_l2
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -88,12 +88,11 @@ def test_numpy_twin_agrees(case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32x"])
 @pytest.mark.parametrize("case", CASES, ids=IDS)
 def test_hip_matches_fixture(gpu, case, prec):
     name, shape, bounds, prm, psi0, V, out = case
-    FT = np.float64 if prec == "f64" else np.float32
-    got = gpu_run(gpu, np.ascontiguousarray(psi0), bounds, V=V, FT=FT, **prm)
+    got = gpu_run(gpu, np.ascontiguousarray(psi0), bounds, V=V, prec=prec, **prm)
     err = rel_l2(got, out)
     print(f"{name} {prec}: {err:.3e}")
-    assert err < (1e-12 if prec == "f64" else 2e-5)
+    assert err < TOLS[prec]

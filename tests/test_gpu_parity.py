@@ -10,7 +10,7 @@ import math
 import numpy as np
 import pytest
 
-from helpers import gpu_run, mesh, random_potential, rel_l2, smooth_state
+from helpers import PRECS, TOLS, gpu_run, mesh, random_potential, rel_l2, smooth_state
 
 pytestmark = pytest.mark.gpu
 
@@ -51,7 +51,7 @@ CASES = [
 
 
 @pytest.mark.parametrize("order", [1, 2])
-@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32x"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[0])) if isinstance(c, tuple) else None)
 def test_split_parity(gpu, orc, case, prec, order):
     shape, bounds, Om, withV = case
@@ -59,14 +59,13 @@ def test_split_parity(gpu, orc, case, prec, order):
     V = random_potential(shape, bounds) if withV else None
     kw = dict(V=V, coeffD=-0.5, beta=3.0, Omega=Om, dt=2e-3, nsteps=10, order=order)
     ref = orc.split_step(psi.copy(), bounds, **kw)
-    FT = np.float64 if prec == "f64" else np.float32
-    out = gpu_run(gpu, psi, bounds, FT=FT, **kw)
+    out = gpu_run(gpu, psi, bounds, prec=prec, **kw)
     err = rel_l2(out, ref)
     print(f"{shape} {prec} order {order}: rel L2 = {err:.3e}")
-    assert err < (F64_TOL if prec == "f64" else F32_TOL)
+    assert err < TOLS[prec]
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32x"])
 @pytest.mark.parametrize("case", [CASES[1], CASES[8]], ids=["2d", "3d"])
 def test_imag_time_parity(gpu, orc, case, prec):
     shape, bounds, Om, _ = case
@@ -74,11 +73,10 @@ def test_imag_time_parity(gpu, orc, case, prec):
     V = random_potential(shape, bounds, amp=10.0)
     kw = dict(V=V, coeffD=-0.5, beta=20.0, Omega=Om, dt=5e-3, nsteps=25, order=2, imag=True)
     ref = orc.split_step(psi.copy(), bounds, **kw)
-    FT = np.float64 if prec == "f64" else np.float32
-    out = gpu_run(gpu, psi, bounds, FT=FT, **kw)
+    out = gpu_run(gpu, psi, bounds, prec=prec, **kw)
     err = rel_l2(out, ref)
     print(f"imag {shape} {prec}: rel L2 = {err:.3e}")
-    assert err < (F64_TOL if prec == "f64" else F32_TOL)
+    assert err < TOLS[prec]
 
 
 def test_thomas_fermi_1000_steps_f64(gpu, orc):
@@ -105,7 +103,7 @@ VARIANT_CASES = [
 ]
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32x"])
 @pytest.mark.parametrize("case", VARIANT_CASES, ids=lambda c: "x".join(map(str, c[0])))
 def test_every_launch_variant(gpu, orc, case, prec):
     """Every compiled sweep variant (line split, workgroup shape) of every
@@ -115,8 +113,8 @@ def test_every_launch_variant(gpu, orc, case, prec):
     V = random_potential(shape, bounds)
     kw = dict(V=V, coeffD=-0.5, beta=3.0, Omega=0.4, dt=2e-3, nsteps=2, order=2)
     ref = orc.split_step(psi.copy(), bounds, **kw)
-    FT = np.float64 if prec == "f64" else np.float32
-    g = gpu.Grid(shape, bounds, FT=FT)
+    FT, compute = PRECS[prec]
+    g = gpu.Grid(shape, bounds, FT=FT, compute=compute)
     f = gpu.Field(g)
     pot = gpu.PotentialArray(f, V)
     p = gpu.GrossPitaevskiiParameters(pot=pot, coeffΔ=-0.5, β=3.0, Ω=0.4)
@@ -137,8 +135,8 @@ def test_every_launch_variant(gpu, orc, case, prec):
             mi.steps(2)
             erri = rel_l2(f.to_numpy(), ref_imag)
             print(f"{shape} {prec} axis {a} variant {v} {ctx.variant(a)[2]}: {err:.3e} imag {erri:.3e}")
-            assert err < (F64_TOL if prec == "f64" else F32_TOL)
-            assert erri < (F64_TOL if prec == "f64" else F32_TOL)
+            assert err < TOLS[prec]
+            assert erri < TOLS[prec]
 
 
 def test_autotune_keeps_results(gpu, orc):
