@@ -113,11 +113,34 @@ int sfd_autotune(sfd_ctx* ctx, void* scratch, float* times_ms);
 
 /* Current launch variant of an axis: its index, the number of variants
  * compiled for this line length and precision, and (optional) its shape
- * {n1, n2, ts, g, pipelined}. */
-int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* shape5);
+ * {n1, n2, ts, g, pipelined, minimum waves per SIMD}. */
+int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* shape6);
 /* Force a launch variant (index < count); results are identical up to
  * floating-point rounding. */
 int sfd_ctx_set_variant(sfd_ctx* ctx, int axis, int index);
+
+/* Batched imaginary-time ground states (no single counterpart in the
+ * reference, which solves one field at a time with NumModelBackwardEuler,
+ * backward-euler.jl:72-90): ninst independent 2D problems of possibly
+ * different extents n[2*i], n[2*i+1] (powers of two, 2..1024) on boxes
+ * box[4*i..4*i+3] = xmin, xmax, ymin, ymax. */
+typedef struct sfd_batch sfd_batch;
+int sfd_batch_create(sfd_batch** out, int ninst, const int64_t* n, const double* box, int precision,
+                     void* stream);
+int sfd_batch_destroy(sfd_batch* b);
+
+/* Advance every field psi[i] (V[i] may be NULL; V itself may be NULL) with
+ * the imaginary-time Strang split (exp(coeffD xi^2 dt/2) sweeps, then
+ * exp(-(V + beta[i]|psi|^2) dt), renormalised to norm 1 after each step in
+ * FP64), until ||psi_{n+1} - psi_n|| < tol (checked every check_every steps)
+ * or max_steps.  Fields of equal extents share each launch; a chunk of
+ * check_every steps is captured once in a hipGraph and replayed when
+ * use_graph != 0.  On return every field is normalised; steps_out[i] and
+ * dpsi_out[i] (optional) receive the steps taken and the last ||Delta psi||.
+ * Synchronises. */
+int sfd_batch_imag_run(sfd_batch* b, void* const* psi, const void* const* V, const double* beta,
+                       double coeffD, double dt, int64_t max_steps, double tol, int check_every,
+                       int use_graph, int64_t* steps_out, double* dpsi_out);
 
 /* Convenience memory helpers on the context's device and stream. */
 int sfd_alloc(sfd_ctx* ctx, size_t bytes, void** out);

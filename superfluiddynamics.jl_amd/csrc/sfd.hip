@@ -16,6 +16,7 @@
 
 #include "../../include/sfd.h"
 #include "sfd_variants.hpp"
+#include "sfd_batch.hpp"
 
 using namespace sfd;
 
@@ -635,6 +636,309 @@ void release(sfd_ctx* c)
 
 }  // namespace
 
+// ------------------------------------------------------------ batch solver
+struct sfd_batch {
+    struct Inst {
+        int n[2];
+        double lo[2], hi[2], d[2];
+        long long nsite;
+        int group;
+        int user;  // index in the caller's ordering
+    };
+    struct Group {
+        int n[2];
+        int first = 0, count = 0;
+        VariantInfo vi[2] = {};
+        void* tw[2] = {nullptr, nullptr};
+        long long ntiles[2] = {0, 0};
+    };
+    int ninst = 0;
+    int prec = SFD_F32;
+    int device = 0;
+    hipStream_t user_stream = nullptr;
+    hipStream_t stream = nullptr;  // own stream (graph capture needs a non-default stream)
+    hipEvent_t ev = nullptr;
+    std::vector<Inst> inst;        // internal (group-major) order
+    std::vector<Group> groups;
+    BatchInstDev* d_inst = nullptr;
+    double* d_scale = nullptr;
+    double* d_snap_scale = nullptr;
+    double* d_dpsi = nullptr;
+    int* d_done = nullptr;
+    long long* d_steps = nullptr;
+    double* d_partial = nullptr;
+    double* d_dpart = nullptr;
+    void* d_snap = nullptr;
+    SweepArgs* d_desc = nullptr;   // [4 kinds][ninst]
+    std::vector<void*> phase;      // per instance, per axis
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_k = 0;
+    static constexpr int kDiffBlocks = 64;
+};
+
+namespace {
+
+template <typename P>
+int batch_setup(sfd_batch* b)
+{
+    using R = typename P::R;
+    using S = typename P::S;
+    // groups of equal extents, internal order group-major
+    std::vector<sfd_batch::Inst> sorted;
+    for (auto& in : b->inst) {
+        int gidx = -1;
+        for (size_t g = 0; g < b->groups.size(); ++g)
+            if (b->groups[g].n[0] == in.n[0] && b->groups[g].n[1] == in.n[1]) gidx = (int)g;
+        if (gidx < 0) {
+            sfd_batch::Group G;
+            G.n[0] = in.n[0];
+            G.n[1] = in.n[1];
+            b->groups.push_back(G);
+            gidx = (int)b->groups.size() - 1;
+        }
+        in.group = gidx;
+    }
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        b->groups[g].first = (int)sorted.size();
+        for (auto& in : b->inst)
+            if (in.group == (int)g) sorted.push_back(in);
+        b->groups[g].count = (int)sorted.size() - b->groups[g].first;
+    }
+    b->inst = sorted;
+    const int ni = b->ninst;
+    long long parts = 0, sites = 0;
+    for (auto& G : b->groups) {
+        for (int ax = 0; ax < 2; ++ax) {
+            if (!variant_info<P>(G.n[ax], ax == 0, 0, &G.vi[ax]))
+                return fail(SFD_ERR_SHAPE, "no sweep variant for line length %d", G.n[ax]);
+            std::vector<cx<R>> w;
+            fill_twiddles<R>(w, G.n[ax], G.vi[ax].n1, G.vi[ax].n2);
+            SFD_HIP(hipMalloc(&G.tw[ax], w.size() * sizeof(cx<R>)));
+            SFD_TRY(upload_sync(G.tw[ax], w.data(), w.size() * sizeof(cx<R>)));
+            const int NL = G.vi[ax].ts * G.vi[ax].g;
+            G.ntiles[ax] = ax == 0 ? (G.n[1] + NL - 1) / NL : (G.n[0] + NL - 1) / NL;
+        }
+        parts += G.ntiles[1] * G.count;
+    }
+    for (auto& in : b->inst) sites += in.nsite;
+    SFD_HIP(hipMalloc(&b->d_inst, ni * sizeof(BatchInstDev)));
+    SFD_HIP(hipMalloc(&b->d_scale, ni * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_snap_scale, ni * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_dpsi, ni * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_done, ni * sizeof(int)));
+    SFD_HIP(hipMalloc(&b->d_steps, ni * sizeof(long long)));
+    SFD_HIP(hipMalloc(&b->d_partial, parts * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_dpart, (size_t)ni * sfd_batch::kDiffBlocks * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_snap, (size_t)sites * sizeof(cx<S>)));
+    SFD_HIP(hipMalloc(&b->d_desc, 4 * (size_t)ni * sizeof(SweepArgs)));
+    b->phase.assign(2 * (size_t)ni, nullptr);
+    return SFD_OK;
+}
+
+// Descriptors and phase tables of one solve (fields, potentials, beta, dt).
+template <typename P>
+int batch_bind(sfd_batch* b, void* const* psi, const void* const* V, const double* beta, double cD, double dt)
+{
+    using R = typename P::R;
+    using S = typename P::S;
+    const int ni = b->ninst;
+    std::vector<BatchInstDev> hi(ni);
+    std::vector<SweepArgs> desc(4 * (size_t)ni);
+    long long poff = 0, soff = 0;
+    const double tau = 0.5 * dt;
+    for (int i = 0; i < ni; ++i) {
+        const auto& in = b->inst[i];
+        const auto& G = b->groups[in.group];
+        if (!psi[in.user]) return fail(SFD_ERR_NULL, "batch: field %d is null", in.user);
+        hi[i].psi = psi[in.user];
+        hi[i].snap = static_cast<char*>(b->d_snap) + soff * sizeof(cx<S>);
+        hi[i].partial = b->d_partial + poff;
+        hi[i].nsite = in.nsite;
+        hi[i].nparts = G.ntiles[1];
+        hi[i].dv = in.d[0] * in.d[1];
+        // imaginary-time multipliers exp(cD xi^2 tau)/n of both axes
+        for (int ax = 0; ax < 2; ++ax) {
+            const int n = in.n[ax];
+            std::vector<cx<R>> h(n);
+            for (int k = 0; k < n; ++k) {
+                const double xi = fft_freq(n, in.d[ax], k);
+                h[k].re = (R)(std::exp((cD * (xi * xi)) * tau) * (1.0 / (double)n));
+                h[k].im = (R)0;
+            }
+            void*& dev = b->phase[2 * (size_t)i + ax];
+            if (!dev) SFD_HIP(hipMalloc(&dev, n * sizeof(cx<R>)));
+            SFD_TRY(upload_sync(dev, h.data(), n * sizeof(cx<R>)));
+        }
+        for (int k = 0; k < 4; ++k) {
+            const int ax = k % 2;
+            SweepArgs a;
+            std::memset(&a, 0, sizeof a);
+            a.psi = psi[in.user];
+            a.V = V ? V[in.user] : nullptr;
+            a.phase = b->phase[2 * (size_t)i + ax];
+            a.tw = G.tw[ax];
+            a.scale = b->d_scale + i;
+            a.beta = beta[in.user];
+            a.dt = dt;
+            if (ax == 0) {
+                a.S = 1;
+                a.nlines = in.n[1];
+            } else {
+                a.S = in.n[0];
+                a.OS = (long long)in.n[0] * in.n[1];
+                a.ncols = in.n[0];
+                a.ntx = (int)G.ntiles[1];
+            }
+            a.ntiles = G.ntiles[ax];
+            // x (rescale), y (+ nonlinear), x, y (+ norm partials)
+            if (k == 0) a.flags = kFlagScale;
+            if (k == 1) a.flags = kFlagNL | kFlagImag;
+            if (k == 3) {
+                a.flags = kFlagNorm;
+                a.partial = b->d_partial + poff;
+            }
+            desc[(size_t)k * ni + i] = a;
+        }
+        poff += G.ntiles[1];
+        soff += in.nsite;
+    }
+    SFD_TRY(upload_sync(b->d_inst, hi.data(), hi.size() * sizeof(BatchInstDev)));
+    SFD_TRY(upload_sync(b->d_desc, desc.data(), desc.size() * sizeof(SweepArgs)));
+    return SFD_OK;
+}
+
+// Enqueue k imaginary-time Strang steps of every unfinished instance; the
+// last step is a convergence check when `check` is set.
+template <typename P>
+int batch_enqueue(sfd_batch* b, int k, bool check, double tol)
+{
+    using S = typename P::S;
+    const int ni = b->ninst;
+    for (int s = 0; s < k; ++s) {
+        const bool chk = check && s == k - 1;
+        if (chk) {
+            hipLaunchKernelGGL(k_batch_snapshot<S>, dim3(sfd_batch::kDiffBlocks, ni), dim3(256), 0, b->stream,
+                               b->d_inst, b->d_scale, b->d_snap_scale, b->d_done);
+            SFD_HIP(hipGetLastError());
+        }
+        for (int kind = 0; kind < 4; ++kind) {
+            const int ax = kind % 2;
+            for (auto& G : b->groups) {
+                const VariantInfo& vi = G.vi[ax];
+                dim3 grid((unsigned)G.ntiles[ax], 1, (unsigned)G.count);
+                dim3 block(vi.ts * vi.g * vi.n2);
+                if (!launch_batch<P>(G.n[ax], ax == 0, 0, grid, block, b->stream,
+                                     b->d_desc + (size_t)kind * ni + G.first, b->d_done + G.first))
+                    return fail(SFD_ERR_SHAPE, "batch: no kernel for line length %d", G.n[ax]);
+                SFD_HIP(hipGetLastError());
+            }
+        }
+        hipLaunchKernelGGL(k_batch_finalize, dim3(ni), dim3(256), 0, b->stream, b->d_inst, b->d_scale, b->d_done,
+                           b->d_steps);
+        SFD_HIP(hipGetLastError());
+        if (chk) {
+            hipLaunchKernelGGL(k_batch_diff<S>, dim3(sfd_batch::kDiffBlocks, ni), dim3(256), 0, b->stream,
+                               b->d_inst, b->d_scale, b->d_snap_scale, b->d_done, b->d_dpart);
+            SFD_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_batch_diff_finalize, dim3(ni), dim3(256), 0, b->stream, b->d_dpart,
+                               sfd_batch::kDiffBlocks, b->d_inst, b->d_dpsi, b->d_done, tol);
+            SFD_HIP(hipGetLastError());
+        }
+    }
+    return SFD_OK;
+}
+
+void batch_drop_graph(sfd_batch* b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->graph_k = 0;
+}
+
+template <typename P>
+int batch_run(sfd_batch* b, void* const* psi, const void* const* V, const double* beta, double cD, double dt,
+              int64_t max_steps, double tol, int check_every, int use_graph, int64_t* steps_out, double* dpsi_out)
+{
+    using S = typename P::S;
+    const int ni = b->ninst;
+    SFD_TRY(batch_bind<P>(b, psi, V, beta, cD, dt));
+    // order after the caller's pending work
+    SFD_HIP(hipEventRecord(b->ev, b->user_stream));
+    SFD_HIP(hipStreamWaitEvent(b->stream, b->ev, 0));
+    hipLaunchKernelGGL(k_batch_reset, dim3((ni + 255) / 256), dim3(256), 0, b->stream, b->d_scale, b->d_done,
+                       b->d_steps, b->d_dpsi, ni, 1);
+    SFD_HIP(hipGetLastError());
+    batch_drop_graph(b);
+    std::vector<int> done(ni, 0);
+    int64_t left = max_steps;
+    while (left > 0) {
+        const int k = (int)(left < check_every ? left : check_every);
+        if (use_graph && k == check_every && k > 0) {
+            if (!b->gexec) {
+                SFD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                int rc = batch_enqueue<P>(b, k, true, tol);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(b->stream, &g);
+                if (rc != SFD_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (e != hipSuccess) return fail(SFD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+                b->graph = g;
+                SFD_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+                b->graph_k = k;
+            }
+            SFD_HIP(hipGraphLaunch(b->gexec, b->stream));
+        } else {
+            SFD_TRY(batch_enqueue<P>(b, k, true, tol));
+        }
+        left -= k;
+        SFD_HIP(hipMemcpyAsync(done.data(), b->d_done, ni * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        SFD_HIP(hipStreamSynchronize(b->stream));
+        bool all = true;
+        for (int i = 0; i < ni; ++i) all = all && done[i];
+        if (all) break;
+    }
+    // apply the pending renormalisation, leave normalised fields
+    hipLaunchKernelGGL(k_batch_apply_scale<S>, dim3(sfd_batch::kDiffBlocks, ni), dim3(256), 0, b->stream, b->d_inst,
+                       b->d_scale);
+    SFD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_batch_reset, dim3((ni + 255) / 256), dim3(256), 0, b->stream, b->d_scale, b->d_done,
+                       b->d_steps, b->d_dpsi, ni, 0);
+    SFD_HIP(hipGetLastError());
+    std::vector<long long> st(ni);
+    std::vector<double> dp(ni);
+    SFD_HIP(hipMemcpyAsync(st.data(), b->d_steps, ni * sizeof(long long), hipMemcpyDeviceToHost, b->stream));
+    SFD_HIP(hipMemcpyAsync(dp.data(), b->d_dpsi, ni * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    SFD_HIP(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < ni; ++i) {
+        if (steps_out) steps_out[b->inst[i].user] = st[i];
+        if (dpsi_out) dpsi_out[b->inst[i].user] = dp[i];
+    }
+    return SFD_OK;
+}
+
+void batch_release(sfd_batch* b)
+{
+    batch_drop_graph(b);
+    for (auto& G : b->groups)
+        for (int ax = 0; ax < 2; ++ax)
+            if (G.tw[ax]) (void)hipFree(G.tw[ax]);
+    for (void* p : b->phase)
+        if (p) (void)hipFree(p);
+    void* ptrs[] = {b->d_inst, b->d_scale, b->d_snap_scale, b->d_dpsi, b->d_done, b->d_steps,
+                    b->d_partial, b->d_dpart, b->d_snap, b->d_desc};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->ev) (void)hipEventDestroy(b->ev);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+}  // namespace
+
 // Call FN<precision traits>(args...) for the context's precision.
 #define SFD_BY_PREC(ctx, FN, ...)                                         \
     ((ctx)->prec == SFD_F32 ? FN<PF32>(__VA_ARGS__)                       \
@@ -811,6 +1115,84 @@ int sfd_fft_axis(sfd_ctx* ctx, const void* in, void* out, int axis, int directio
     return SFD_BY_PREC(ctx, fft_impl, ctx, in, out, axis, direction);
 }
 
+int sfd_batch_create(sfd_batch** out, int ninst, const int64_t* n, const double* box, int precision, void* stream)
+{
+    g_last_error.clear();
+    if (!out || !n || !box) return fail(SFD_ERR_NULL, "sfd_batch_create: null argument");
+    *out = nullptr;
+    if (ninst < 1) return fail(SFD_ERR_ARG, "sfd_batch_create: ninst must be >= 1");
+    if (precision != SFD_F32 && precision != SFD_F64 && precision != SFD_F32X)
+        return fail(SFD_ERR_ARG, "precision must be SFD_F32, SFD_F64 or SFD_F32X (got %d)", precision);
+    auto* b = new sfd_batch();
+    b->ninst = ninst;
+    b->prec = precision;
+    b->user_stream = static_cast<hipStream_t>(stream);
+    for (int i = 0; i < ninst; ++i) {
+        sfd_batch::Inst in;
+        in.nsite = 1;
+        for (int a = 0; a < 2; ++a) {
+            const int64_t m = n[2 * i + a];
+            if (m < 2 || m > 1024 || (m & (m - 1))) {
+                delete b;
+                return fail(SFD_ERR_SHAPE, "instance %d axis %d: n = %lld is not a power of two in [2, 1024]", i, a,
+                            (long long)m);
+            }
+            in.n[a] = (int)m;
+            in.lo[a] = box[4 * i + 2 * a];
+            in.hi[a] = box[4 * i + 2 * a + 1];
+            if (!(in.hi[a] > in.lo[a])) {
+                delete b;
+                return fail(SFD_ERR_ARG, "instance %d axis %d: bounds must satisfy min < max", i, a);
+            }
+            in.d[a] = (in.hi[a] - in.lo[a]) / (double)m;
+            in.nsite *= m;
+        }
+        in.group = -1;
+        in.user = i;
+        b->inst.push_back(in);
+    }
+    hipError_t e = hipGetDevice(&b->device);
+    if (e == hipSuccess) e = hipStreamCreate(&b->stream);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        batch_release(b);
+        delete b;
+        return fail(SFD_ERR_HIP, "sfd_batch_create: %s", hipGetErrorString(e));
+    }
+    int rc = SFD_BY_PREC(b, batch_setup, b);
+    if (rc != SFD_OK) {
+        batch_release(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return SFD_OK;
+}
+
+int sfd_batch_destroy(sfd_batch* b)
+{
+    if (!b) return SFD_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    batch_release(b);
+    delete b;
+    return SFD_OK;
+}
+
+int sfd_batch_imag_run(sfd_batch* b, void* const* psi, const void* const* V, const double* beta, double coeffD,
+                       double dt, int64_t max_steps, double tol, int check_every, int use_graph, int64_t* steps_out,
+                       double* dpsi_out)
+{
+    if (!b) return fail(SFD_ERR_NULL, "null batch");
+    if (!psi || !beta) return fail(SFD_ERR_NULL, "sfd_batch_imag_run: null argument");
+    if (max_steps < 0 || check_every < 1) return fail(SFD_ERR_ARG, "max_steps >= 0 and check_every >= 1 required");
+    if (!std::isfinite(coeffD) || !std::isfinite(dt) || !std::isfinite(tol))
+        return fail(SFD_ERR_ARG, "non-finite scalar argument");
+    SFD_HIP(hipSetDevice(b->device));
+    return SFD_BY_PREC(b, batch_run, b, psi, V, beta, coeffD, dt, max_steps, tol, check_every, use_graph,
+                       steps_out, dpsi_out);
+}
+
 int sfd_autotune(sfd_ctx* ctx, void* scratch, float* times_ms)
 {
     SFD_TRY(check_ctx(ctx));
@@ -819,18 +1201,19 @@ int sfd_autotune(sfd_ctx* ctx, void* scratch, float* times_ms)
     return SFD_BY_PREC(ctx, autotune_impl, ctx, scratch, times_ms);
 }
 
-int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* shape5)
+int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* shape6)
 {
     if (!ctx || !index || !count) return fail(SFD_ERR_NULL, "sfd_ctx_get_variant: null argument");
     if (axis < 0 || axis >= ctx->ndim) return fail(SFD_ERR_ARG, "axis %d out of range", axis);
     *index = ctx->var[axis];
     *count = SFD_BY_PREC(ctx, n_variants, ctx->n[axis], axis == 0);
-    if (shape5) {
-        shape5[0] = ctx->vi[axis].n1;
-        shape5[1] = ctx->vi[axis].n2;
-        shape5[2] = ctx->vi[axis].ts;
-        shape5[3] = ctx->vi[axis].g;
-        shape5[4] = ctx->vi[axis].pipe;
+    if (shape6) {
+        shape6[0] = ctx->vi[axis].n1;
+        shape6[1] = ctx->vi[axis].n2;
+        shape6[2] = ctx->vi[axis].ts;
+        shape6[3] = ctx->vi[axis].g;
+        shape6[4] = ctx->vi[axis].pipe;
+        shape6[5] = ctx->vi[axis].waves;
     }
     return SFD_OK;
 }

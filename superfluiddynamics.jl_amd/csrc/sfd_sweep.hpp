@@ -306,8 +306,10 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
 }
 
 // One launch over all tiles of an axis, one tile per workgroup; every mode.
-template <typename S, typename R, int N1, int N2, int TS, int G, int MODE>
-__global__ __launch_bounds__(TS * G * N2) void k_sweep(const SweepArgs a)
+// W is the minimum number of wavefronts per SIMD the register allocation
+// must leave room for (1 = no constraint).
+template <typename S, typename R, int N1, int N2, int TS, int G, int MODE, int W = 1>
+__global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
 {
     using Sh = SweepShape<R, N1, N2, TS, G>;
     constexpr int N = Sh::N;
@@ -400,6 +402,47 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep(const SweepArgs a)
         }
         block_partials<R, Sh::NT, 5>(acc, a.partial, nblocks, tile);
     }
+}
+
+// Batched round trip over independent fields: blockIdx.z selects the
+// instance, whose descriptor (field, potential, tables, strides, flags,
+// pending scale, partial sums) sits in descs[]; instances whose done flag is
+// set are skipped.  All instances of one launch share N1, N2 and the
+// workgroup shape; grid.x*grid.y covers the largest tile count.
+template <typename S, typename R, int N1, int N2, int TS, int G>
+__global__ __launch_bounds__(TS * G * N2) void k_batch_sweep(const SweepArgs* __restrict__ descs,
+                                                             const int* __restrict__ done)
+{
+    using Sh = SweepShape<R, N1, N2, TS, G>;
+    constexpr int N = Sh::N;
+    constexpr int NL = Sh::NL;
+    __shared__ cx<R> lds[Sh::LDS];
+    const int inst = blockIdx.z;
+    if (done[inst]) return;
+    const SweepArgs a = descs[inst];
+    const long long tile = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+    if (tile >= a.ntiles) return;
+
+    const int tid = threadIdx.x;
+    const int c = tid % TS;
+    const int t = (tid / TS) % N2;
+    const int g = tid / (TS * N2);
+    const int L = c + TS * g;
+    const TileGeom gm = tile_geom<N, N2, TS, NL>(a, tile, L, t);
+    const cx<R>* __restrict__ twA = static_cast<const cx<R>*>(a.tw) + t;
+    const cx<R>* __restrict__ twB = static_cast<const cx<R>*>(a.tw) + N + t;
+    cx<S>* __restrict__ psi = static_cast<cx<S>*>(a.psi);
+    cx<R> v[N1];
+    R vp[N1];
+    load_line<S, R, N1>(v, psi, gm);
+    if (a.flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
+    double nacc = 0.0;
+    round_trip_tile<R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
+    if (a.flags & kFlagNorm) {
+        double acc1[1] = {nacc};
+        block_partials<R, Sh::NT, 1>(acc1, a.partial, a.ntiles, tile);
+    }
+    store_line<S, R, N1>(v, psi, gm);
 }
 
 // Persistent, software-pipelined round trip: each workgroup walks tiles

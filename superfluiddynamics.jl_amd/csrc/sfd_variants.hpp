@@ -10,14 +10,15 @@
 namespace sfd {
 
 struct VariantInfo {
-    int n1, n2, ts, g, pipe;
+    int n1, n2, ts, g, pipe, waves;
 };
 
 // pipe = 1: the round trip runs as the persistent, software-pipelined
 // k_sweep_pipe (the other modes use k_sweep with the same shape).
-template <int N1_, int N2_, int TS_, int G_, int P_ = 0>
+// W: minimum wavefronts per SIMD the register allocation must allow.
+template <int N1_, int N2_, int TS_, int G_, int P_ = 0, int W_ = 1>
 struct V {
-    static constexpr int N1 = N1_, N2 = N2_, TS = TS_, G = G_, P = P_;
+    static constexpr int N1 = N1_, N2 = N2_, TS = TS_, G = G_, P = P_, W = W_;
 };
 template <typename... Vs>
 struct VL {
@@ -68,13 +69,9 @@ SFD_VARIANTS(PF32, 32, (VL<V<8, 4, 1, 64>>), (VL<V<8, 4, 16, 1>>))
 SFD_VARIANTS(PF32, 64, (VL<V<8, 8, 1, 32>, V<16, 4, 1, 64>>), (VL<V<8, 8, 16, 1>, V<8, 8, 32, 1>>))
 SFD_VARIANTS(PF32, 128, (VL<V<16, 8, 1, 32>, V<16, 8, 1, 16>, V<32, 4, 1, 32>, V<16, 8, 1, 16, 1>>),
              (VL<V<16, 8, 16, 1>, V<16, 8, 32, 1>, V<32, 4, 32, 1>, V<16, 8, 16, 1, 1>>))
-SFD_VARIANTS(PF32, 256,
-             (VL<V<16, 16, 1, 16>, V<16, 16, 1, 8>, V<32, 8, 1, 16>, V<32, 8, 1, 8>, V<16, 16, 1, 16, 1>,
-                 V<16, 16, 1, 8, 1>>),
-             (VL<V<16, 16, 16, 1>, V<16, 16, 32, 1>, V<32, 8, 16, 1>, V<32, 8, 32, 1>, V<16, 16, 16, 1, 1>,
-                 V<16, 16, 8, 1, 1>>))
-SFD_VARIANTS(PF32, 512, (VL<V<32, 16, 1, 8>, V<32, 16, 1, 4>, V<32, 16, 1, 4, 1>>),
-             (VL<V<32, 16, 16, 1>, V<32, 16, 8, 1>, V<32, 16, 8, 1, 1>>))
+SFD_VARIANTS(PF32, 256, (VL<V<16, 16, 1, 16>, V<16, 16, 1, 8>, V<32, 8, 1, 16>, V<16, 16, 1, 16, 1>>),
+             (VL<V<16, 16, 16, 1>, V<16, 16, 32, 1>, V<32, 8, 16, 1>, V<16, 16, 8, 1, 1>>))
+SFD_VARIANTS(PF32, 512, (VL<V<32, 16, 1, 8>, V<32, 16, 1, 4>>), (VL<V<32, 16, 16, 1>, V<32, 16, 8, 1>>))
 SFD_VARIANTS(PF32, 1024, (VL<V<32, 32, 1, 4>, V<32, 32, 1, 2>>), (VL<V<32, 32, 8, 1>, V<32, 32, 4, 1>>))
 
 // ---------------------------------------------------------------- FP64
@@ -86,8 +83,7 @@ SFD_VARIANTS(PF64, 32, (VL<V<8, 4, 1, 64>>), (VL<V<8, 4, 16, 1>>))
 SFD_VARIANTS(PF64, 64, (VL<V<8, 8, 1, 32>, V<16, 4, 1, 32>>), (VL<V<8, 8, 16, 1>, V<8, 8, 32, 1>>))
 SFD_VARIANTS(PF64, 128, (VL<V<16, 8, 1, 16>, V<16, 8, 1, 8>, V<32, 4, 1, 16>>),
              (VL<V<16, 8, 16, 1>, V<16, 8, 32, 1>, V<32, 4, 16, 1>>))
-SFD_VARIANTS(PF64, 256, (VL<V<16, 16, 1, 8>, V<16, 16, 1, 4>, V<16, 16, 1, 8, 1>, V<16, 16, 1, 4, 1>>),
-             (VL<V<16, 16, 16, 1>, V<16, 16, 8, 1>, V<16, 16, 16, 1, 1>, V<16, 16, 8, 1, 1>>))
+SFD_VARIANTS(PF64, 256, (VL<V<16, 16, 1, 8>, V<16, 16, 1, 4>>), (VL<V<16, 16, 16, 1>, V<16, 16, 8, 1>>))
 SFD_VARIANTS(PF64, 512, (VL<V<32, 16, 1, 4>>), (VL<V<32, 16, 8, 1>, V<32, 16, 4, 1>>))
 SFD_VARIANTS(PF64, 1024, (VL<V<32, 32, 1, 2>>), (VL<V<32, 32, 4, 1>, V<32, 32, 2, 1>>))
 
@@ -105,6 +101,7 @@ template <typename P, int MODE>
 struct Dispatch<P, MODE, VL<>> {
     static bool info(int, VariantInfo*) { return false; }
     static bool launch(int, dim3, dim3, hipStream_t, const SweepArgs&) { return false; }
+    static bool launch_batch(int, dim3, dim3, hipStream_t, const SweepArgs*, const int*) { return false; }
     static int occupancy(int) { return 0; }
 };
 
@@ -115,7 +112,7 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
     static bool info(int idx, VariantInfo* out)
     {
         if (idx == 0) {
-            *out = {V0::N1, V0::N2, V0::TS, V0::G, V0::P};
+            *out = {V0::N1, V0::N2, V0::TS, V0::G, V0::P, V0::W};
             return true;
         }
         return Dispatch<P, MODE, VL<Vs...>>::info(idx - 1, out);
@@ -126,10 +123,21 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
             if constexpr (V0::P && MODE == kRoundTrip)
                 hipLaunchKernelGGL((k_sweep_pipe<S, R, V0::N1, V0::N2, V0::TS, V0::G>), grid, block, 0, s, a);
             else
-                hipLaunchKernelGGL((k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE>), grid, block, 0, s, a);
+                hipLaunchKernelGGL((k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W>), grid, block, 0, s,
+                                   a);
             return true;
         }
         return Dispatch<P, MODE, VL<Vs...>>::launch(idx - 1, grid, block, s, a);
+    }
+    static bool launch_batch(int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs* descs,
+                             const int* done)
+    {
+        if (idx == 0) {
+            hipLaunchKernelGGL((k_batch_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G>), grid, block, 0, s, descs,
+                               done);
+            return true;
+        }
+        return Dispatch<P, MODE, VL<Vs...>>::launch_batch(idx - 1, grid, block, s, descs, done);
     }
     // resident workgroups per CU of the persistent round-trip kernel
     static int occupancy(int idx)
@@ -204,6 +212,24 @@ bool variant_info(int n, bool contig, int idx, VariantInfo* out)
 {
 #define SFD_CASE(N_) \
     if (n == N_) return info_variant_n<P, N_>(contig, idx, out);
+    SFD_FOR_EACH_N(SFD_CASE)
+#undef SFD_CASE
+    return false;
+}
+
+template <typename P, int N>
+bool launch_batch_n(bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs* d, const int* done)
+{
+    if (contig) return Dispatch<P, 0, typename Variants<P, N>::contig>::launch_batch(idx, grid, block, s, d, done);
+    return Dispatch<P, 0, typename Variants<P, N>::strided>::launch_batch(idx, grid, block, s, d, done);
+}
+
+template <typename P>
+bool launch_batch(int n, bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs* d,
+                  const int* done)
+{
+#define SFD_CASE(N_) \
+    if (n == N_) return launch_batch_n<P, N_>(contig, idx, grid, block, s, d, done);
     SFD_FOR_EACH_N(SFD_CASE)
 #undef SFD_CASE
     return false;

@@ -153,9 +153,9 @@ class _Ctx:
         return {a: [t for t in times[a * 8:(a + 1) * 8] if t >= 0] for a in range(3) if times[a * 8] >= 0}
 
     def variant(self, axis: int):
-        """(index, count, (n1, n2, ts, g, pipelined)) of the sweep variant used on axis."""
+        """(index, count, (n1, n2, ts, g, pipelined, min waves/SIMD)) of the sweep variant on axis."""
         idx, cnt = ctypes.c_int(), ctypes.c_int()
-        shp = (ctypes.c_int * 5)()
+        shp = (ctypes.c_int * 6)()
         L.check(self.lib.sfd_ctx_get_variant(self.h, axis, ctypes.byref(idx), ctypes.byref(cnt), shp))
         return idx.value, cnt.value, tuple(shp)
 
@@ -463,3 +463,65 @@ def grid_x(f: Field):
 
 grid_y = grid_x
 grid_z = grid_x
+
+
+# --------------------------------------------------- batched ground states
+class BatchImagSolver:
+    """Many independent 2D imaginary-time ground-state solves advanced
+    together (sfd_batch_imag_run): fields of equal extents share every sweep
+    launch, each field keeps its own β, V, FP64 renormalisation factor, step
+    count and convergence flag, and chunks of steps replay as a hipGraph.
+
+    ``fields`` is a list of 2D Fields of one precision; ``params`` a list of
+    GrossPitaevskiiParameters (one coeffΔ for all, Ω = 0)."""
+
+    def __init__(self, fields, params, Δt):
+        _require_gpu()
+        if len(fields) != len(params) or not fields:
+            raise L.ArgError(L.SFD_ERR_ARG, "one parameter set per field")
+        g0 = fields[0].g
+        for f in fields:
+            if f.g.ndims != 2:
+                raise L.ArgError(L.SFD_ERR_ARG, "batched solves are 2D")
+            if (f.g.FT, f.g.compute) != (g0.FT, g0.compute):
+                raise L.ArgError(L.SFD_ERR_ARG, "all fields of a batch share one precision")
+        cds = {p.coeffΔ for p in params}
+        if len(cds) != 1 or any(p.Ω != 0.0 for p in params):
+            raise L.ArgError(L.SFD_ERR_ARG, "batched solves need one coeffΔ and Ω = 0")
+        self.fields, self.params, self.Δt = list(fields), list(params), float(Δt)
+        self.coeffΔ = cds.pop()
+        lib = L.load()
+        self.lib = lib
+        ni = len(fields)
+        n = (ctypes.c_int64 * (2 * ni))(*[v for f in fields for v in f.g.n])
+        box = (ctypes.c_double * (4 * ni))(*[v for f in fields for a in range(2) for v in (f.g.min[a], f.g.max[a])])
+        if g0.FT is np.float64:
+            prec = L.SFD_F64
+        else:
+            prec = L.SFD_F32X if g0.compute is np.float64 else L.SFD_F32
+        h = ctypes.c_void_p()
+        L.check(lib.sfd_batch_create(ctypes.byref(h), ni, n, box, prec,
+                                     ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        self.h = h
+
+    def run(self, max_steps=20000, tol=1e-9, check_every=100, use_graph=True):
+        """Returns (steps taken per field, last ||Δψ|| per field)."""
+        ni = len(self.fields)
+        psi = (ctypes.c_void_p * ni)(*[f.ϕ.data_ptr() for f in self.fields])
+        vps = [p.pot.device_ptr() if p.pot is not None else None for p in self.params]
+        V = (ctypes.c_void_p * ni)(*[v if v else None for v in vps])
+        beta = (ctypes.c_double * ni)(*[p.β for p in self.params])
+        steps = (ctypes.c_int64 * ni)()
+        dpsi = (ctypes.c_double * ni)()
+        L.check(self.lib.sfd_batch_imag_run(self.h, psi, V, beta, self.coeffΔ, self.Δt, int(max_steps), float(tol),
+                                            int(check_every), int(bool(use_graph)), steps, dpsi))
+        return np.array(steps[:], dtype=np.int64), np.array(dpsi[:])
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.sfd_batch_destroy(h)
+            except Exception:
+                pass
+            self.h = None

@@ -72,3 +72,25 @@ def metric_params(n=256):
 
 def sha256(a) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def batch_ground_states(seed=0, counts=((128, 32), (256, 48), (512, 16)), L=10.0):
+    """96 uneven 2D ground-state problems on [-L, L)²: V = ½(x² + γ²y²) with
+    γ ~ U[1, 3], β ~ U[100, 2000], and a seeded Gaussian initial guess
+    exp(-(x²/σx² + y²/σy²)/2) with σx, σy ~ U[1, 3], normalised.
+    Returns a list of dicts (n, gamma, beta, psi0, V, bounds)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for n, count in counts:
+        x = coords(n, -L, L)
+        X, Y = x[None, :], x[:, None]
+        d = (2 * L / n, 2 * L / n)
+        for _ in range(count):
+            gamma = rng.uniform(1.0, 3.0)
+            beta = rng.uniform(100.0, 2000.0)
+            sx, sy = rng.uniform(1.0, 3.0, size=2)
+            psi = np.exp(-0.5 * (X ** 2 / sx ** 2 + Y ** 2 / sy ** 2)) + 0j
+            psi = np.ascontiguousarray(discrete_normalize(psi, d))
+            V = np.ascontiguousarray(0.5 * (X ** 2 + gamma ** 2 * Y ** 2) + 0 * X)
+            out.append(dict(n=n, gamma=gamma, beta=beta, psi0=psi, V=V, bounds=((-L, L), (-L, L))))
+    return out
