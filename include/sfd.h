@@ -142,6 +142,25 @@ int sfd_batch_imag_run(sfd_batch* b, void* const* psi, const void* const* V, con
                        double coeffD, double dt, int64_t max_steps, double tol, int check_every,
                        int use_graph, int64_t* steps_out, double* dpsi_out);
 
+/* Adjoint of the real-time Strang split (no counterpart in the reference):
+ * loss L = 1 - |<target|psi_N>|^2, <a|b> = sum conj(a) b dV, after nsteps
+ * NumModelSplit2 steps from psi0 (psi0 is not modified).  Returns *loss and
+ * dL/dbeta on the host and writes dL/dV(x) (FP64, nsites values) to the
+ * device buffer gradV when non-NULL; psi_final (optional) receives psi_N.
+ * The reverse sweep recomputes forward states from checkpoints taken every
+ * checkpoint_every steps (<= 0: ceil(sqrt(nsteps))).  Synchronises. */
+int sfd_adjoint(sfd_ctx* ctx, const void* psi0, const void* V, const void* target, double coeffD,
+                double beta, double Omega, double dt, int64_t nsteps, int64_t checkpoint_every,
+                double* gradV, double* grad_beta, double* loss, void* psi_final);
+
+/* <a|b> = sum conj(a) b dV (FP64, out2 = {re, im}).  Synchronises. */
+int sfd_inner(sfd_ctx* ctx, const void* a, const void* b, double* out2);
+
+/* out[j] = sum_x field[x] * basis[j*nsites + x] (FP64 device arrays; the
+ * chain rule from dL/dV to the coefficients of V = sum_j a_j basis_j).
+ * Synchronises. */
+int sfd_project(sfd_ctx* ctx, const double* field, const double* basis, int nbasis, double* out);
+
 /* Convenience memory helpers on the context's device and stream. */
 int sfd_alloc(sfd_ctx* ctx, size_t bytes, void** out);
 int sfd_free(sfd_ctx* ctx, void* ptr);

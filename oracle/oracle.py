@@ -52,6 +52,14 @@ def lib():
         L.orc_energy.restype = ctypes.c_int
         L.orc_axis_tables.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _c_dbl_p, _c_dbl_p]
         L.orc_axis_tables.restype = ctypes.c_int
+        L.orc_adjoint.argtypes = [ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int64,
+                                  _c_dbl_p, _c_dbl_p, _c_dbl_p]
+        L.orc_adjoint.restype = ctypes.c_int
+        L.orc_overlap_loss.argtypes = [ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int64]
+        L.orc_overlap_loss.restype = ctypes.c_double
         L.orc_num_threads.argtypes = []
         L.orc_num_threads.restype = ctypes.c_int
         _lib = L
@@ -136,6 +144,32 @@ def axis_tables(n, lo, hi):
     k = np.empty(n)
     lib().orc_axis_tables(int(n), float(lo), float(hi), c.ctypes.data_as(_c_dbl_p), k.ctypes.data_as(_c_dbl_p))
     return c, k
+
+
+def adjoint(psi0, target, bounds, *, V=None, coeffD=-0.5, beta=1.0, Omega=0.0, dt=1e-3, nsteps=1):
+    """(loss, dL/dV field, dL/dbeta) of L = 1 - |<target|psi_N>|^2 after nsteps
+    real-time Strang steps (reverse sweep, all states stored)."""
+    psi0 = np.ascontiguousarray(psi0, dtype=np.complex128)
+    target = np.ascontiguousarray(target, dtype=np.complex128)
+    nd, n, lo, hi = _geom(_xfirst(psi0), bounds)
+    Vk, vp = _vbuf(V, psi0)
+    gV = np.zeros(psi0.shape)
+    gb = ctypes.c_double()
+    loss = ctypes.c_double()
+    rc = lib().orc_adjoint(nd, n, lo, hi, _cbuf(psi0), vp, _cbuf(target), coeffD, beta, Omega, dt, int(nsteps),
+                           gV.ctypes.data_as(_c_dbl_p), ctypes.byref(gb), ctypes.byref(loss))
+    if rc != 0:
+        raise ValueError(f"orc_adjoint failed ({rc})")
+    return loss.value, gV, gb.value
+
+
+def overlap_loss(psi0, target, bounds, *, V=None, coeffD=-0.5, beta=1.0, Omega=0.0, dt=1e-3, nsteps=1):
+    psi0 = np.ascontiguousarray(psi0, dtype=np.complex128)
+    target = np.ascontiguousarray(target, dtype=np.complex128)
+    nd, n, lo, hi = _geom(_xfirst(psi0), bounds)
+    Vk, vp = _vbuf(V, psi0)
+    return lib().orc_overlap_loss(nd, n, lo, hi, _cbuf(psi0), vp, _cbuf(target), coeffD, beta, Omega, dt,
+                                  int(nsteps))
 
 
 def num_threads() -> int:

@@ -487,6 +487,114 @@ int orc_energy(int ndim, const int *n, const double *lo, const double *hi,
     return 0;
 }
 
+/* ------------------------------------------------------------- adjoint ---
+ * Reverse-mode gradient of L = 1 - |<target|psi_N>|^2, <a|b> = sum conj(a) b dV,
+ * through nsteps real-time Strang steps psi1 = K psi_n, psi2 = exp(-i th) psi1
+ * with th = (V + beta |psi1|^2) dt, psi_{n+1} = K psi2 (splitting.jl:149-209).
+ * Gradient convention: G = dL/du + i dL/dv for psi = u + i v, so that
+ * dL = Re(conj(G) dpsi).  Backward step:
+ *   G <- K^dagger G   (sweeps over -tau in reverse axis order)
+ *   w = exp(i th) G, q = Im(conj(w) psi1),
+ *   G <- w + 2 beta dt q psi1,  dL/dV(x) += q dt,  dL/dbeta += q dt |psi1|^2
+ *   G <- K^dagger G.
+ * Terminal: G_N = -2 c dV target, c = <target|psi_N>.
+ * All pre-nonlinear states are stored (small test problems only). */
+static void lap_rot_adj(const geom *g, cplx *G, double cD, double Om, double tau)
+{
+    for (int ax = g->ndim - 1; ax >= 0; --ax) sweep_axis(g, G, ax, cD, Om, -tau, 0);
+}
+
+int orc_adjoint(int ndim, const int *n, const double *lo, const double *hi,
+                const double *psi0_ri, const double *V, const double *target_ri,
+                double cD, double beta, double Om, double dt, int64_t nsteps,
+                double *gradV, double *grad_beta, double *loss)
+{
+    if (ndim < 1 || ndim > 3 || nsteps < 0) return -3;
+    geom g;
+    geom_init(&g, ndim, n, lo, hi);
+    const int64_t ns = g.nsite;
+    const double dv = dvol(&g);
+    const double tau = 0.5 * dt;
+    cplx *psi = (cplx *)malloc(sizeof(cplx) * (size_t)ns);
+    cplx *G = (cplx *)malloc(sizeof(cplx) * (size_t)ns);
+    cplx *store = (cplx *)malloc(sizeof(cplx) * (size_t)ns * (size_t)(nsteps > 0 ? nsteps : 1));
+    const cplx *tgt = (const cplx *)target_ri;
+    memcpy(psi, psi0_ri, sizeof(cplx) * (size_t)ns);
+    for (int64_t s = 0; s < nsteps; ++s) {
+        lap_rot(&g, psi, cD, Om, tau, 0);
+        memcpy(store + (size_t)s * ns, psi, sizeof(cplx) * (size_t)ns);
+        solve_nl(&g, psi, V, beta, dt, 0);
+        lap_rot(&g, psi, cD, Om, tau, 0);
+    }
+    double cr = 0.0, ci = 0.0;
+    for (int64_t i = 0; i < ns; ++i) {
+        cr += tgt[i].re * psi[i].re + tgt[i].im * psi[i].im;
+        ci += tgt[i].re * psi[i].im - tgt[i].im * psi[i].re;
+    }
+    cr *= dv;
+    ci *= dv;
+    *loss = 1.0 - (cr * cr + ci * ci);
+    for (int64_t i = 0; i < ns; ++i) {
+        /* -2 c dV target */
+        G[i].re = -2.0 * dv * (cr * tgt[i].re - ci * tgt[i].im);
+        G[i].im = -2.0 * dv * (cr * tgt[i].im + ci * tgt[i].re);
+        gradV[i] = 0.0;
+    }
+    double gb = 0.0;
+    for (int64_t s = nsteps - 1; s >= 0; --s) {
+        const cplx *p1 = store + (size_t)s * ns;
+        lap_rot_adj(&g, G, cD, Om, tau);
+        for (int64_t i = 0; i < ns; ++i) {
+            const cplx p = p1[i];
+            const double a2 = p.re * p.re + p.im * p.im;
+            const double th = ((V ? V[i] : 0.0) + beta * a2) * dt;
+            const double c = cos(th), sn = sin(th);
+            const cplx w = {c * G[i].re - sn * G[i].im, c * G[i].im + sn * G[i].re};
+            const double q = w.re * p.im - w.im * p.re;
+            G[i].re = w.re + 2.0 * beta * dt * q * p.re;
+            G[i].im = w.im + 2.0 * beta * dt * q * p.im;
+            gradV[i] += q * dt;
+            gb += q * dt * a2;
+        }
+        lap_rot_adj(&g, G, cD, Om, tau);
+    }
+    *grad_beta = gb;
+    free(psi);
+    free(G);
+    free(store);
+    geom_free(&g);
+    return 0;
+}
+
+/* L = 1 - |<target|psi_N>|^2 after nsteps real-time Strang steps. */
+double orc_overlap_loss(int ndim, const int *n, const double *lo, const double *hi,
+                        const double *psi0_ri, const double *V, const double *target_ri,
+                        double cD, double beta, double Om, double dt, int64_t nsteps)
+{
+    geom g;
+    geom_init(&g, ndim, n, lo, hi);
+    const int64_t ns = g.nsite;
+    const double dv = dvol(&g);
+    cplx *psi = (cplx *)malloc(sizeof(cplx) * (size_t)ns);
+    memcpy(psi, psi0_ri, sizeof(cplx) * (size_t)ns);
+    for (int64_t s = 0; s < nsteps; ++s) {
+        lap_rot(&g, psi, cD, Om, 0.5 * dt, 0);
+        solve_nl(&g, psi, V, beta, dt, 0);
+        lap_rot(&g, psi, cD, Om, 0.5 * dt, 0);
+    }
+    const cplx *tgt = (const cplx *)target_ri;
+    double cr = 0.0, ci = 0.0;
+    for (int64_t i = 0; i < ns; ++i) {
+        cr += tgt[i].re * psi[i].re + tgt[i].im * psi[i].im;
+        ci += tgt[i].re * psi[i].im - tgt[i].im * psi[i].re;
+    }
+    cr *= dv;
+    ci *= dv;
+    free(psi);
+    geom_free(&g);
+    return 1.0 - (cr * cr + ci * ci);
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

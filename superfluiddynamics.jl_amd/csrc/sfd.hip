@@ -7,6 +7,7 @@
 // workspace.  Fields and potentials stay owned by the caller.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -17,6 +18,7 @@
 #include "../../include/sfd.h"
 #include "sfd_variants.hpp"
 #include "sfd_batch.hpp"
+#include "sfd_adjoint.hpp"
 
 using namespace sfd;
 
@@ -376,7 +378,7 @@ int base_args(sfd_ctx* c, int axis, SweepArgs& a)
 // One linear round trip along `axis` over tau, with optional fused flags.
 template <typename P>
 int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, double cD, double Om, int imag,
-               int flags, double beta, double dt)
+               int flags, double beta, double dt, void* save = nullptr)
 {
     using R = typename P::R;
     using S = typename P::S;
@@ -397,6 +399,7 @@ int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, doubl
     a.beta = beta;
     a.dt = dt;
     a.scale = c->scal;
+    a.save = save;
     if (flags & kFlagNorm) {
         SFD_TRY(ensure_partial(c, 5 * sweep_nblocks(c, axis)));
         a.partial = c->partial;
@@ -535,6 +538,114 @@ int fft_impl(sfd_ctx* c, const void* in, void* out, int axis, int dir)
     a.out = out;
     if (dir == SFD_FORWARD) return launch_sweep<P, kForward>(c, axis, a);
     return launch_sweep<P, kInverse>(c, axis, a);
+}
+
+// One real-time Strang step that also stores the state before the
+// nonlinear phase (psi1 = K psi) in `save`.
+template <typename P>
+int strang_step_save(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt, void* save)
+{
+    const int D = c->ndim;
+    const double tau = 0.5 * dt;
+    for (int h = 0; h < 2; ++h)
+        for (int ax = 0; ax < D; ++ax) {
+            const bool nl = (h == 0 && ax == D - 1);
+            SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, nl ? kFlagNL : 0, beta, dt, nl ? save : nullptr));
+        }
+    return SFD_OK;
+}
+
+// K^dagger: round trips over -tau in reverse axis order.
+template <typename P>
+int lap_rot_adjoint(sfd_ctx* c, void* G, double cD, double Om, double tau)
+{
+    for (int ax = c->ndim - 1; ax >= 0; --ax) SFD_TRY(round_trip<P>(c, G, nullptr, ax, -tau, cD, Om, 0, 0, 0.0, 0.0));
+    return SFD_OK;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// Gradient of L = 1 - |<target|psi_N>|^2 through nsteps real-time Strang
+// steps by the reverse sweep, recomputing forward states from checkpoints
+// taken every `every` steps (sqrt(nsteps) when every <= 0): memory is
+// (nsteps/every + every) fields, forward work twice the plain run.
+template <typename P>
+int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target, double cD, double beta, double Om,
+                 double dt, int64_t N, int64_t every, double* gradV, double* grad_beta, double* loss, void* psi_final)
+{
+    using S = typename P::S;
+    using R = typename P::R;
+    const long long ns = c->nsite;
+    const size_t fb = (size_t)ns * sizeof(cx<S>);
+    if (every <= 0) every = (int64_t)std::ceil(std::sqrt((double)(N > 0 ? N : 1)));
+    const int64_t nck = N > 0 ? (N + every - 1) / every : 0;
+    const long long nb = grid_blocks(ns);
+    DevBuf psi, G, ck, seg, part, acc;
+    SFD_HIP(hipMalloc(&psi.p, fb));
+    SFD_HIP(hipMalloc(&G.p, fb));
+    if (nck > 0) SFD_HIP(hipMalloc(&ck.p, fb * (size_t)nck));
+    SFD_HIP(hipMalloc(&seg.p, fb * (size_t)(every > 0 ? every : 1)));
+    SFD_HIP(hipMalloc(&part.p, 2 * (size_t)nb * sizeof(double)));
+    SFD_HIP(hipMalloc(&acc.p, 4 * sizeof(double)));
+    auto fld = [&](void* base, int64_t k) { return static_cast<void*>(static_cast<char*>(base) + fb * (size_t)k); };
+    double* accd = static_cast<double*>(acc.p);
+    double* partd = static_cast<double*>(part.p);
+
+    // forward pass with checkpoints
+    SFD_HIP(hipMemcpyAsync(psi.p, psi0, fb, hipMemcpyDeviceToDevice, c->stream));
+    for (int64_t k = 0; k < nck; ++k) {
+        SFD_HIP(hipMemcpyAsync(fld(ck.p, k), psi.p, fb, hipMemcpyDeviceToDevice, c->stream));
+        const int64_t m = std::min<int64_t>(every, N - k * every);
+        SFD_TRY(split_impl<P>(c, psi.p, V, cD, beta, Om, dt, m, 2, SFD_REAL_TIME));
+    }
+    if (psi_final) SFD_HIP(hipMemcpyAsync(psi_final, psi.p, fb, hipMemcpyDeviceToDevice, c->stream));
+
+    // loss and terminal gradient
+    hipLaunchKernelGGL(k_inner<S>, dim3((unsigned)nb), dim3(256), 0, c->stream, static_cast<const cx<S>*>(target),
+                       static_cast<const cx<S>*>(psi.p), ns, partd);
+    SFD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, partd, nb, 2, accd, 0.0, (double*)nullptr);
+    SFD_HIP(hipGetLastError());
+    double cs[2];
+    SFD_HIP(hipMemcpyAsync(cs, accd, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SFD_HIP(hipStreamSynchronize(c->stream));
+    const double dv = c->dvol();
+    const double cr = cs[0] * dv, ci = cs[1] * dv;
+    if (loss) *loss = 1.0 - (cr * cr + ci * ci);
+    hipLaunchKernelGGL(k_terminal<S>, dim3((unsigned)nb), dim3(256), 0, c->stream, static_cast<cx<S>*>(G.p),
+                       static_cast<const cx<S>*>(target), ns, (const double*)accd, dv);
+    SFD_HIP(hipGetLastError());
+    if (gradV) SFD_HIP(hipMemsetAsync(gradV, 0, (size_t)ns * sizeof(double), c->stream));
+    SFD_HIP(hipMemsetAsync(accd + 2, 0, sizeof(double), c->stream));
+
+    // reverse sweep, segment by segment from the last checkpoint
+    const double tau = 0.5 * dt;
+    for (int64_t k = nck - 1; k >= 0; --k) {
+        const int64_t m = std::min<int64_t>(every, N - k * every);
+        SFD_HIP(hipMemcpyAsync(psi.p, fld(ck.p, k), fb, hipMemcpyDeviceToDevice, c->stream));
+        for (int64_t j = 0; j < m; ++j) SFD_TRY(strang_step_save<P>(c, psi.p, V, cD, beta, Om, dt, fld(seg.p, j)));
+        for (int64_t j = m - 1; j >= 0; --j) {
+            SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
+            hipLaunchKernelGGL((k_nl_adjoint<S, R>), dim3((unsigned)nb), dim3(256), 0, c->stream,
+                               static_cast<cx<S>*>(G.p), static_cast<const cx<S>*>(fld(seg.p, j)),
+                               static_cast<const S*>(V), ns, (R)beta, (R)dt, gradV, partd);
+            SFD_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(256), 0, c->stream, (const double*)partd, nb, accd + 2);
+            SFD_HIP(hipGetLastError());
+            SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
+        }
+    }
+    double gb = 0.0;
+    SFD_HIP(hipMemcpyAsync(&gb, accd + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SFD_HIP(hipStreamSynchronize(c->stream));
+    if (grad_beta) *grad_beta = gb;
+    return SFD_OK;
 }
 
 // Autotuner: for every axis with more than one compiled variant, time the
@@ -1191,6 +1302,68 @@ int sfd_batch_imag_run(sfd_batch* b, void* const* psi, const void* const* V, con
     SFD_HIP(hipSetDevice(b->device));
     return SFD_BY_PREC(b, batch_run, b, psi, V, beta, coeffD, dt, max_steps, tol, check_every, use_graph,
                        steps_out, dpsi_out);
+}
+
+int sfd_adjoint(sfd_ctx* ctx, const void* psi0, const void* V, const void* target, double coeffD, double beta,
+                double Omega, double dt, int64_t nsteps, int64_t checkpoint_every, double* gradV, double* grad_beta,
+                double* loss, void* psi_final)
+{
+    SFD_TRY(check_ctx(ctx));
+    if (!psi0 || !target) return fail(SFD_ERR_NULL, "sfd_adjoint: null field");
+    if (nsteps < 0) return fail(SFD_ERR_ARG, "nsteps must be >= 0");
+    if (!std::isfinite(coeffD) || !std::isfinite(beta) || !std::isfinite(Omega) || !std::isfinite(dt))
+        return fail(SFD_ERR_ARG, "non-finite scalar argument");
+    return SFD_BY_PREC(ctx, adjoint_impl, ctx, psi0, V, target, coeffD, beta, Omega, dt, nsteps, checkpoint_every,
+                       gradV, grad_beta, loss, psi_final);
+}
+
+int sfd_inner(sfd_ctx* ctx, const void* a, const void* b, double* out2)
+{
+    SFD_TRY(check_ctx(ctx));
+    if (!a || !b || !out2) return fail(SFD_ERR_NULL, "sfd_inner: null argument");
+    const long long nb = grid_blocks(ctx->nsite);
+    SFD_TRY(ensure_partial(ctx, 2 * nb));
+    if (ctx->prec == SFD_F64)
+        hipLaunchKernelGGL(k_inner<double>, dim3((unsigned)nb), dim3(256), 0, ctx->stream,
+                           static_cast<const cx<double>*>(a), static_cast<const cx<double>*>(b), ctx->nsite,
+                           ctx->partial);
+    else
+        hipLaunchKernelGGL(k_inner<float>, dim3((unsigned)nb), dim3(256), 0, ctx->stream,
+                           static_cast<const cx<float>*>(a), static_cast<const cx<float>*>(b), ctx->nsite,
+                           ctx->partial);
+    SFD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, ctx->stream, ctx->partial, nb, 2, ctx->scal + 4, 0.0,
+                       (double*)nullptr);
+    SFD_HIP(hipGetLastError());
+    SFD_HIP(hipMemcpyAsync(out2, ctx->scal + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SFD_HIP(hipStreamSynchronize(ctx->stream));
+    const double dv = ctx->dvol();
+    out2[0] *= dv;
+    out2[1] *= dv;
+    return SFD_OK;
+}
+
+int sfd_project(sfd_ctx* ctx, const double* field, const double* basis, int nbasis, double* out)
+{
+    SFD_TRY(check_ctx(ctx));
+    if (!field || !basis || !out) return fail(SFD_ERR_NULL, "sfd_project: null argument");
+    if (nbasis < 1 || nbasis > 65535) return fail(SFD_ERR_ARG, "nbasis must be in [1, 65535]");
+    const long long nb = 64;
+    DevBuf part, res;
+    SFD_HIP(hipMalloc(&part.p, (size_t)nb * nbasis * sizeof(double)));
+    SFD_HIP(hipMalloc(&res.p, (size_t)nbasis * sizeof(double)));
+    hipLaunchKernelGGL(k_project, dim3((unsigned)nb, (unsigned)nbasis), dim3(256), 0, ctx->stream, field, basis,
+                       ctx->nsite, static_cast<double*>(part.p));
+    SFD_HIP(hipGetLastError());
+    for (int j = 0; j < nbasis; ++j) {
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, ctx->stream,
+                           static_cast<const double*>(part.p) + (size_t)j * nb, nb, 1,
+                           static_cast<double*>(res.p) + j, 0.0, (double*)nullptr);
+        SFD_HIP(hipGetLastError());
+    }
+    SFD_HIP(hipMemcpyAsync(out, res.p, (size_t)nbasis * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SFD_HIP(hipStreamSynchronize(ctx->stream));
+    return SFD_OK;
 }
 
 int sfd_autotune(sfd_ctx* ctx, void* scratch, float* times_ms)

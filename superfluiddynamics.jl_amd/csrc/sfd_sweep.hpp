@@ -62,6 +62,7 @@ struct SweepArgs {
     double dt;              // NL time step
     long long ntiles;       // workgroup tiles of the launch
     int ntx;                // strided flavour: x-tiles per "other" index
+    void* save;             // kFlagNL: the state before the nonlinear phase is also stored here
 };
 
 template <typename R, int N1, int N2, int TS, int G>
@@ -267,7 +268,7 @@ __device__ __forceinline__ void inverse_ba(cx<R> (&w)[N1 / N2][N2], cx<R> (&v)[N
 
 // The round trip of one tile in registers: forward, phase multiply,
 // inverse, then the fused epilogue (nonlinear phase, norm partial sum).
-template <typename R, int N1, int N2, int TS, int G>
+template <typename S, typename R, int N1, int N2, int TS, int G>
 __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1], const TileGeom& gm,
                                                 const SweepArgs& a, cx<R>* lds, const cx<R>* twA,
                                                 const cx<R>* twB, int c, int g, int t, double& norm_acc)
@@ -289,6 +290,7 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
         for (int k2 = 0; k2 < N2; ++k2) w[r][k2] = cmul(w[r][k2], ph[N2 * r + N1 * k2]);
     inverse_ba<R, N1, N2, TS, G>(w, v, lds, twB, c, g, t);
     if (flags & kFlagNL) {
+        if (a.save) store_line<S, R, N1>(v, static_cast<cx<S>*>(a.save), gm);
         const R beta = R(a.beta), dt = R(a.dt);
         const bool imag = flags & kFlagImag;
 #pragma unroll
@@ -337,7 +339,7 @@ __global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
         load_line<S, R, N1>(v, psi, gm);
         if (flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
         double nacc = 0.0;
-        round_trip_tile<R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
+        round_trip_tile<S, R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
         if (flags & kFlagNorm) {
             double acc1[1] = {nacc};
             block_partials<R, Sh::NT, 1>(acc1, a.partial, nblocks, tile);
@@ -437,7 +439,7 @@ __global__ __launch_bounds__(TS * G * N2) void k_batch_sweep(const SweepArgs* __
     load_line<S, R, N1>(v, psi, gm);
     if (a.flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
     double nacc = 0.0;
-    round_trip_tile<R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
+    round_trip_tile<S, R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
     if (a.flags & kFlagNorm) {
         double acc1[1] = {nacc};
         block_partials<R, Sh::NT, 1>(acc1, a.partial, a.ntiles, tile);
@@ -489,7 +491,7 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
             load_line<S, R, N1>(vn, psi, gn);
             if (nl) load_pot<S, R, N1>(vpn, V, gn);
         }
-        round_trip_tile<R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
+        round_trip_tile<S, R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
         store_line<S, R, N1>(v, psi, gm);
         if (!more) break;
         __syncthreads();  // every lane is done with the exchange buffer

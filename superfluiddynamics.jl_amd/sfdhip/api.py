@@ -525,3 +525,47 @@ class BatchImagSolver:
             except Exception:
                 pass
             self.h = None
+
+
+# ------------------------------------------------------------- adjoint
+def inner(f: Field, a, b) -> complex:
+    """<a|b> = Σ conj(a) b ΠΔ (FP64) for tensors on f's grid."""
+    c = f.ctx()
+    out = (ctypes.c_double * 2)()
+    L.check(c.lib.sfd_inner(c.h, ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), out))
+    return complex(out[0], out[1])
+
+
+def adjoint_gradient(n: _SplitBase, target, nsteps: int, checkpoint_every: int = 0, want_state: bool = False):
+    """Reverse-mode gradient of L = 1 - |<target|ψ(T)>|² through ``nsteps``
+    real-time Strang steps of model ``n`` from its current field (which is
+    left unchanged).  Returns (L, dL/dV as an FP64 device tensor, dL/dβ[, ψ(T)]).
+    Forward states are recomputed from √N checkpoints (or every
+    ``checkpoint_every`` steps)."""
+    if n.mode != L.SFD_REAL_TIME or n.order != 2:
+        raise L.ArgError(L.SFD_ERR_ARG, "the adjoint is defined for the real-time Strang split")
+    f = n.f
+    c = f.ctx()
+    p = n.param
+    tgt = target.ϕ if isinstance(target, Field) else target
+    gV = torch.zeros(f.g.shape, dtype=torch.float64, device="cuda")
+    gb, loss = ctypes.c_double(), ctypes.c_double()
+    out = torch.empty_like(f.ϕ) if want_state else None
+    L.check(c.lib.sfd_adjoint(c.h, ctypes.c_void_p(f.ϕ.data_ptr()), _vptr(p), ctypes.c_void_p(tgt.data_ptr()),
+                              p.coeffΔ, p.β, p.Ω, n.Δt, int(nsteps), int(checkpoint_every),
+                              ctypes.c_void_p(gV.data_ptr()), ctypes.byref(gb), ctypes.byref(loss),
+                              ctypes.c_void_p(out.data_ptr()) if want_state else None))
+    res = (loss.value, gV, gb.value)
+    return res + (out,) if want_state else res
+
+
+def project(f: Field, field, basis) -> np.ndarray:
+    """out[j] = Σ_x field[x]·basis[j, x] (FP64 device tensors; basis shape
+    (nb,) + grid shape): the chain rule from dL/dV to V = Σ_j a_j basis_j."""
+    c = f.ctx()
+    field = field.to(torch.float64).contiguous()
+    basis = basis.to(torch.float64).contiguous()
+    nb = basis.shape[0]
+    out = (ctypes.c_double * nb)()
+    L.check(c.lib.sfd_project(c.h, ctypes.c_void_p(field.data_ptr()), ctypes.c_void_p(basis.data_ptr()), nb, out))
+    return np.array(out[:])

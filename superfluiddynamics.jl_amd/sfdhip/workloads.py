@@ -94,3 +94,26 @@ def batch_ground_states(seed=0, counts=((128, 32), (256, 48), (512, 16)), L=10.0
             V = np.ascontiguousarray(0.5 * (X ** 2 + gamma ** 2 * Y ** 2) + 0 * X)
             out.append(dict(n=n, gamma=gamma, beta=beta, psi0=psi, V=V, bounds=((-L, L), (-L, L))))
     return out
+
+
+def control_problem(n=256, L=8.0, seed=0, sigma2=0.5, lattice=8, a_std=0.1):
+    """Differentiable-control setup on [-L, L)²: V(a) = ½|x|² +
+    Σ_j a_j exp(-|x - c_j|²/σ²) with c_j on a lattice×lattice grid of cell
+    centres and a_j ~ N(0, a_std²) from the seed.  Returns (basis (nb, n, n),
+    a, V0 = ½|x|², bounds)."""
+    x = coords(n, -L, L)
+    X, Y = x[None, :], x[:, None]
+    h = 2 * L / lattice
+    centres = [(-L + (i + 0.5) * h, -L + (j + 0.5) * h) for j in range(lattice) for i in range(lattice)]
+    basis = np.stack([np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / sigma2) for cx, cy in centres])
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a = rng.normal(0.0, a_std, size=len(centres))
+    V0 = np.ascontiguousarray(0.5 * (X ** 2 + Y ** 2) + 0 * X)
+    return np.ascontiguousarray(basis), a, V0, ((-L, L), (-L, L))
+
+
+def shift_x(psi, shift, L):
+    """Band-limited shift ψ(x - shift) along x (the last numpy axis)."""
+    n = psi.shape[-1]
+    k = 2 * np.pi * np.fft.fftfreq(n, 2 * L / n)
+    return np.fft.ifft(np.fft.fft(psi, axis=-1) * np.exp(-1j * k * shift), axis=-1)
