@@ -115,6 +115,37 @@ def cpu_leg(psi0, mp, cpu_steps):
     return t, oracle.num_threads()
 
 
+def traffic_driver(n=256, launches=5):
+    """Launches for the rocprofv3 --pmc runs: `launches` round trips of each
+    axis of the 256³ FP32 workload (x: contiguous kernel, then y and z: the
+    strided kernel), then `launches` in-place passes of a plain elementwise
+    kernel over the same 128 MiB (torch mul_), the calibration whose
+    algorithmic traffic is exactly one read and one write of the field."""
+    import ctypes
+    import numpy as np
+    import torch
+    import sfdhip as sfd
+    from sfdhip.workloads import metric_params, random_phase_3d
+    mp = metric_params(n)
+    g = sfd.Grid((n, n, n), mp["bounds"], FT=np.float32)
+    f = sfd.Field(g)
+    f.set(random_phase_3d(n, seed=0))
+    lib, ctx = sfd.load(), f.ctx()
+    psi = ctypes.c_void_p(f.ϕ.data_ptr())
+    for ax in range(3):
+        for _ in range(launches):
+            rc = lib.sfd_lap_rot_axis(ctx.h, psi, ax, mp["coeffD"], 0.0, mp["dt"] * 0.5, 0)
+            assert rc == 0, lib.sfd_last_error()
+        torch.cuda.synchronize()
+        print(f"[traffic] axis {ax}: {launches} launches, variant {ctx.variant(ax)[2]}", flush=True)
+    flat = torch.view_as_real(f.ϕ)
+    for _ in range(launches):
+        flat.mul_(1.0)
+    torch.cuda.synchronize()
+    print(f"[traffic] calibration: {launches} in-place passes over {flat.numel() * 4} bytes", flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,7 +156,11 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-launches", type=int, default=40)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--traffic-driver", action="store_true",
+                    help="only launch the kernels profiled by the rocprofv3 --pmc traffic runs")
     args = ap.parse_args(argv)
+    if args.traffic_driver:
+        return traffic_driver(args.n)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the whole {args.n}³ domain is resident on a single MI355X; "
               f"--gpus {args.gpus} is not supported (the slab-decomposed multi-GPU path is not built).",
