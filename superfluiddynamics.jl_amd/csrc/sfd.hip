@@ -181,6 +181,8 @@ namespace {
 // Twiddles in the layout k_sweep reads, w = exp(-2 pi i / n):
 //   [0, n):   twA[s][t] = w^(s t), s < n1, t < n2
 //   [n, 2n):  twB[q][s] = w^(q s), q < n2, s < n1
+// FP64 entries are (cos, sin) of the angle a; FP32 entries are the lifting
+// coefficients (tan(b/2), sin b) of b = a/2 with a taken in (-pi, pi].
 template <typename R>
 void fill_twiddles(std::vector<cx<R>>& w, int n, int n1, int n2)
 {
@@ -188,8 +190,15 @@ void fill_twiddles(std::vector<cx<R>>& w, int n, int n1, int n2)
     const long double pi = 3.14159265358979323846264338327950288L;
     auto put = [&](size_t i, long long m) {
         const long double a = -2.0L * pi * (long double)(m % n) / (long double)n;
-        w[i].re = (R)cosl(a);
-        w[i].im = (R)sinl(a);
+        if constexpr (sizeof(R) == 4) {
+            // lifting coefficients of the half angle b in (-pi/2, pi/2]
+            const long double b = 0.5L * (2 * (m % n) > n ? a + 2.0L * pi : a);
+            w[i].re = (R)tanl(0.5L * b);
+            w[i].im = (R)sinl(b);
+        } else {
+            w[i].re = (R)cosl(a);
+            w[i].im = (R)sinl(a);
+        }
     };
     for (int s = 0; s < n1; ++s)
         for (int t = 0; t < n2; ++t) put((size_t)s * n2 + t, (long long)t * s);
@@ -201,6 +210,15 @@ int upload_sync(void* dst, const void* src, size_t bytes)
 {
     SFD_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return SFD_OK;
+}
+
+// One imaginary-time multiplier exp(arg)/n: itself in FP64 tables, and
+// expm1(arg)/n in FP32 tables, whose sweeps form X/n + X expm1(arg)/n.
+template <typename R>
+cx<R> decay_entry(double arg, double inv_n)
+{
+    if constexpr (sizeof(R) == 4) return {(R)(std::expm1(arg) * inv_n), (R)0};
+    return {(R)(std::exp(arg) * inv_n), (R)0};
 }
 
 // Phase multiplier of one axis (splitting.jl:190,195,200), divided by n:
@@ -227,6 +245,7 @@ int build_phase(sfd_ctx* c, int axis, double tau, double cD, double Om, int imag
     const int oax = axis == 0 ? 1 : 0;
     const int rows = rot ? c->n[oax] : 1;
     std::vector<cx<R>> h((size_t)rows * n);
+    std::vector<double> args(sizeof(R) == 4 && !imag ? h.size() : 0);
     const double inv_n = 1.0 / (double)n;
     for (int r = 0; r < rows; ++r) {
         const double co = rot ? lin_coord(c->lo[oax], c->hi[oax], c->n[oax], r) : 0.0;
@@ -236,12 +255,16 @@ int build_phase(sfd_ctx* c, int axis, double tau, double cD, double Om, int imag
             if (axis == 0) arg = (cD * (xi * xi) - (Om * co) * xi) * tau;
             else if (axis == 1) arg = (cD * (xi * xi) + (Om * co) * xi) * tau;
             else arg = (cD * (xi * xi)) * tau;
-            double mr, mi;
-            if (imag) { mr = std::exp(arg); mi = 0.0; }
-            else      { mr = std::cos(arg); mi = std::sin(arg); }
-            h[(size_t)r * n + k].re = (R)(mr * inv_n);
-            h[(size_t)r * n + k].im = (R)(mi * inv_n);
+            if (imag) h[(size_t)r * n + k] = decay_entry<R>(arg, inv_n);
+            else if (sizeof(R) == 4) args[(size_t)r * n + k] = std::remainder(arg, 2.0 * M_PI);
+            else h[(size_t)r * n + k] = {(R)(std::cos(arg) * inv_n), (R)(std::sin(arg) * inv_n)};
         }
+    }
+    // FP32 real time: lifting coefficients (tan(b/2), sin b) of the half
+    // phase b = phi/2, phi in (-pi, pi]; the sweep applies them twice
+    for (size_t i = 0; i < args.size(); ++i) {
+        const double b = 0.5 * args[i];
+        h[i] = {(R)std::tan(0.5 * b), (R)std::sin(b)};
     }
     void* dev = nullptr;
     SFD_HIP(hipMalloc(&dev, h.size() * sizeof(cx<R>)));
@@ -316,7 +339,7 @@ int launch_sweep(sfd_ctx* c, int axis, SweepArgs a)
     using S = typename P::S;
     (void)sizeof(R);
     (void)sizeof(S);
-    Launch L = sweep_geometry(c, axis, a, MODE == kRoundTrip);
+    Launch L = sweep_geometry(c, axis, a, MODE == kRoundTrip || MODE == kRoundTripImag);
     a.tw = c->tw[axis];
     if (!launch_variant<P, MODE>(c->n[axis], axis == 0, c->var[axis], L.grid, L.block, c->stream, a))
         return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", c->var[axis], c->n[axis]);
@@ -395,7 +418,7 @@ int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, doubl
     a.phase_row_stride = rows > 1 ? c->n[axis] : 0;
     if (axis == 0) a.row_mod = rows > 1 ? c->n[1] : 0;
     else a.row_from_col = (axis == 1 && rows > 1) ? 1 : 0;
-    a.flags = flags;
+    a.flags = flags | (imag ? kFlagImag : 0);
     a.beta = beta;
     a.dt = dt;
     a.scale = c->scal;
@@ -404,6 +427,8 @@ int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, doubl
         SFD_TRY(ensure_partial(c, 5 * sweep_nblocks(c, axis)));
         a.partial = c->partial;
     }
+    if constexpr (sizeof(R) == 4)
+        if (imag) return launch_sweep<P, kRoundTripImag>(c, axis, a);
     return launch_sweep<P, kRoundTrip>(c, axis, a);
 }
 
@@ -874,8 +899,7 @@ int batch_bind(sfd_batch* b, void* const* psi, const void* const* V, const doubl
             std::vector<cx<R>> h(n);
             for (int k = 0; k < n; ++k) {
                 const double xi = fft_freq(n, in.d[ax], k);
-                h[k].re = (R)(std::exp((cD * (xi * xi)) * tau) * (1.0 / (double)n));
-                h[k].im = (R)0;
+                h[k] = decay_entry<R>((cD * (xi * xi)) * tau, 1.0 / (double)n);
             }
             void*& dev = b->phase[2 * (size_t)i + ax];
             if (!dev) SFD_HIP(hipMalloc(&dev, n * sizeof(cx<R>)));
@@ -903,10 +927,11 @@ int batch_bind(sfd_batch* b, void* const* psi, const void* const* V, const doubl
             }
             a.ntiles = G.ntiles[ax];
             // x (rescale), y (+ nonlinear), x, y (+ norm partials)
-            if (k == 0) a.flags = kFlagScale;
-            if (k == 1) a.flags = kFlagNL | kFlagImag;
+            a.flags = kFlagImag;
+            if (k == 0) a.flags |= kFlagScale;
+            if (k == 1) a.flags |= kFlagNL;
             if (k == 3) {
-                a.flags = kFlagNorm;
+                a.flags |= kFlagNorm;
                 a.partial = b->d_partial + poff;
             }
             desc[(size_t)k * ni + i] = a;

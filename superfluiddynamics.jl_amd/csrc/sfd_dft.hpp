@@ -8,6 +8,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "sfd_f32_lifting.hpp"
+
 namespace sfd {
 
 template <typename R>
@@ -74,7 +76,28 @@ __device__ __forceinline__ constexpr R cos64(int m)
 template <typename R>
 __device__ __forceinline__ constexpr R sin64(int m) { return cos64<R>(m - 16); }
 
+// Rotation of v by theta as three lifting steps (shears), t = tan(theta/2),
+// s = sin(theta); lift(lift(v, t, s), -t, -s) == v in exact arithmetic for
+// any rounded t and s.
+__device__ __forceinline__ cx<float> lift(cx<float> v, float t, float s)
+{
+    v.re = fmaf(-t, v.im, v.re);
+    v.im = fmaf(s, v.re, v.im);
+    v.re = fmaf(-t, v.im, v.re);
+    return v;
+}
+
 // v * exp(SIGN 2 pi i e / N) for a compile-time-foldable e; N divides 64.
+//
+// FP64 multiplies by the correctly rounded (cos, sin).  FP32 splits
+// e = q N/4 + r with |r| <= N/8: the quarter turns are exact, and the rest is
+// a lifting rotation with coefficients from kLiftT/kLiftS.  The inverse
+// (SIGN = +1) applies the lifting steps with negated coefficients, then the
+// quarter turns, so it undoes the forward rotation of the same e exactly (in
+// exact arithmetic).  The inverse DFTs mirror the forward ones, so an FP32
+// forward-inverse round trip carries no systematic error from the rounding
+// of its constants: a complex multiply by rounded cos/sin(pi/8) instead
+// loses 5.7e-8 of norm on every pass.
 template <int N, int SIGN, typename R>
 __device__ __forceinline__ cx<R> twid(cx<R> v, int e)
 {
@@ -84,21 +107,33 @@ __device__ __forceinline__ cx<R> twid(cx<R> v, int e)
     if (2 * e == N) return {-v.re, -v.im};
     if (4 * e == N) return muli<SIGN>(v);
     if (4 * e == 3 * N) return muli<-SIGN>(v);
-    const int m = e * (64 / N);
-    const R c = cos64<R>(m);
-    const R s = R(SIGN) * sin64<R>(m);
-    if ((8 * e) % N == 0) {
-        // odd multiple of pi/4: |c| = |s| = sqrt(1/2)
-        const R h = R(kQuarterCos[8]);
-        const R sc = c > R(0) ? R(1) : R(-1);
-        const R ss = s > R(0) ? R(1) : R(-1);
-        cx<R> r;
-        if (sc == ss) { r.re = h * (v.re - v.im); r.im = h * (v.im + v.re); }
-        else          { r.re = h * (v.re + v.im); r.im = h * (v.im - v.re); }
-        if (sc < R(0)) { r.re = -r.re; r.im = -r.im; }
-        return r;
+    if constexpr (sizeof(R) == 4) {
+        const int q = (4 * e + N / 2) / N;  // nearest quarter turn
+        const int m = (e - q * (N / 4)) * (64 / N) + 8;
+        const float t = kLiftT[m], s = kLiftS[m];
+        auto quarter = [&](cx<float> x, int sgn) {
+            for (int k = 0; k < (q & 3); ++k) x = sgn < 0 ? muli<-1>(x) : muli<+1>(x);
+            return x;
+        };
+        if constexpr (SIGN < 0) return lift(quarter(v, -1), t, s);
+        else return quarter(lift(v, -t, -s), +1);
+    } else {
+        const int m = e * (64 / N);
+        const R c = cos64<R>(m);
+        const R s = R(SIGN) * sin64<R>(m);
+        if ((8 * e) % N == 0) {
+            // odd multiple of pi/4: |c| = |s| = sqrt(1/2)
+            const R h = R(kQuarterCos[8]);
+            const R sc = c > R(0) ? R(1) : R(-1);
+            const R ss = s > R(0) ? R(1) : R(-1);
+            cx<R> r;
+            if (sc == ss) { r.re = h * (v.re - v.im); r.im = h * (v.im + v.re); }
+            else          { r.re = h * (v.re + v.im); r.im = h * (v.im - v.re); }
+            if (sc < R(0)) { r.re = -r.re; r.im = -r.im; }
+            return r;
+        }
+        return {v.re * c - v.im * s, v.re * s + v.im * c};
     }
-    return {v.re * c - v.im * s, v.re * s + v.im * c};
 }
 
 // In-register DFT of N points (N | 64), natural order in and out.
@@ -118,6 +153,27 @@ __device__ __forceinline__ void dft(cx<R>* v)
         v[2] = s02 - s13;
         v[1] = d02 + d13;
         v[3] = d02 - d13;
+    } else if constexpr (SIGN > 0) {
+        // decimation in frequency: the mirror (adjoint) of the forward
+        // decimation in time below, X[k + M q] -> x[P m + p]
+        constexpr int P = (N % 16 == 0) ? 4 : 2;
+        constexpr int M = N / P;
+        cx<R> sub[P][M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            cx<R> z[P];
+#pragma unroll
+            for (int q = 0; q < P; ++q) z[q] = v[k + M * q];
+            dft<P, SIGN, R>(z);
+#pragma unroll
+            for (int p = 0; p < P; ++p) sub[p][k] = twid<N, SIGN, R>(z[p], p * k);
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            dft<M, SIGN, R>(sub[p]);
+#pragma unroll
+            for (int m = 0; m < M; ++m) v[P * m + p] = sub[p][m];
+        }
     } else {
         // decimation in time, N = P*M: x[P m + p] -> X[k + M q]
         constexpr int P = (N % 16 == 0) ? 4 : 2;
@@ -173,6 +229,43 @@ __device__ __forceinline__ void sincos_cw(float x, float* sp, float* cp)
     const float b = (q & 1) ? sn : cs;
     *sp = (q & 2) ? -a : a;
     *cp = ((q + 1) & 2) ? -b : b;
+}
+
+// sin and cos of |y| <= pi/4 by the Taylor series of sincos_cw (no reduction).
+__device__ __forceinline__ void sincos_small(float y, float* sp, float* cp)
+{
+    const float y2 = y * y;
+    float ps = fmaf(y2, 2.7557319223985893e-6f, -1.984126984126984e-4f);
+    ps = fmaf(y2, ps, 8.333333333333333e-3f);
+    ps = fmaf(y2, ps, -1.6666666666666666e-1f);
+    *sp = fmaf(y * y2, ps, y);
+    float pc = fmaf(y2, -2.755731922398589e-7f, 2.48015873015873e-5f);
+    pc = fmaf(y2, pc, -1.388888888888889e-3f);
+    pc = fmaf(y2, pc, 4.1666666666666664e-2f);
+    pc = fmaf(y2, pc, -0.5f);
+    *cp = fmaf(y2, pc, 1.0f);
+}
+
+// x reduced to [-pi, pi] by x - k 2pi, 2pi split in three parts (Cody-Waite;
+// k 2pi_hi and k 2pi_mid are exact for |k| < 2048); larger |x| reduce in FP64.
+__device__ __forceinline__ float reduce_2pi(float x)
+{
+    if (!(fabsf(x) <= 1.0e4f)) return (float)remainder((double)x, 6.283185307179586476925);
+    const float k = rintf(x * 0x1.45f306p-3f);
+    float r = fmaf(-k, 0x1.92p+2f, x);
+    r = fmaf(-k, 0x1.fb6p-10f, r);
+    return fmaf(-k, -0x1.777a5cp-23f, r);
+}
+
+// Lifting coefficients (t, s) of the rotation by -x/2 (apply twice for the
+// rotation by -x): t = tan(-x/4), s = sin(-x/2), x taken mod 2pi so |t| <= 1.
+__device__ __forceinline__ void lift_half_neg(float x, float* tp, float* sp)
+{
+    const float r = reduce_2pi(x);
+    float sq, cq;
+    sincos_small(0.25f * r, &sq, &cq);
+    *tp = -sq * __builtin_amdgcn_rcpf(cq);
+    *sp = -2.0f * sq * cq;
 }
 
 __device__ __forceinline__ void sincos_cw(double x, double* sp, double* cp)

@@ -29,6 +29,7 @@ enum SweepMode : int {
     kForward = 1,    // forward only, spectrum stored in natural order
     kInverse = 2,    // inverse only (normalised by 1/N), spectrum read in natural order
     kEnergy = 3,     // forward only, spectral and pointwise sums, nothing stored
+    kRoundTripImag = 4,  // kRoundTrip for imaginary time (FP32 sweeps: real decay tables)
 };
 
 enum SweepFlags : int {
@@ -93,6 +94,15 @@ __device__ __forceinline__ void nl_phase(cx<R>& v, R vpot, R beta, R dt, bool im
         else e = exp(-th);
         v.re *= e;
         v.im *= e;
+    } else if constexpr (sizeof(R) == 4) {
+        // exp(-i th) v as two lifting rotations by -th/2.  Every lifting step
+        // adds a first-order increment whose rounding is unbiased; a complex
+        // multiply by (cos th, -sin th) instead drops the second-order part
+        // 1 - cos th whenever it is below half an ulp, the same way at every
+        // site of a near-uniform density, and so drifts the norm.
+        float t, s;
+        lift_half_neg(th, &t, &s);
+        v = lift(lift(v, t, s), t, s);
     } else {
         R s, c;
         sincos_cw(th, &s, &c);
@@ -219,6 +229,25 @@ __device__ __forceinline__ void store_line(const cx<R> (&v)[N1], cx<S>* psi, con
     }
 }
 
+// Inter-stage twiddles.  FP64 tables hold w = exp(-2 pi i e/n) and the
+// inverse multiplies by conj(w).  FP32 tables hold the lifting coefficients
+// (tan(b/2), sin(b)) of the half angle b = arg(w)/2 in (-pi/2, pi/2]: the
+// forward applies that rotation twice and the inverse applies its exact
+// inverse twice, so FP32 round trips keep no systematic error from the
+// rounding of the tables.
+template <typename R>
+__device__ __forceinline__ cx<R> tw_mul(cx<R> v, cx<R> w)
+{
+    if constexpr (sizeof(R) == 4) return lift(lift(v, w.re, w.im), w.re, w.im);
+    else return cmul(v, w);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> tw_mulc(cx<R> v, cx<R> w)
+{
+    if constexpr (sizeof(R) == 4) return lift(lift(v, -w.re, -w.im), -w.re, -w.im);
+    else return cmulc(v, w);
+}
+
 // Forward transform of the tile's lines: phase A (N1-point DFTs, twiddles),
 // LDS exchange, phase B (N2-point DFTs).  On return w[r][k2] holds
 // X[s + N1 k2] of row s = t + N2 r.
@@ -229,7 +258,7 @@ __device__ __forceinline__ void forward_ab(cx<R> (&v)[N1], cx<R> (&w)[N1 / N2][N
     using Sh = SweepShape<R, N1, N2, TS, G>;
     dft<N1, -1, R>(v);
 #pragma unroll
-    for (int s = 1; s < N1; ++s) v[s] = cmul(v[s], twA[s * N2]);
+    for (int s = 1; s < N1; ++s) v[s] = tw_mul(v[s], twA[s * N2]);
 #pragma unroll
     for (int s = 0; s < N1; ++s) lds[Sh::slot(c, g, s, t)] = v[s];
     __syncthreads();
@@ -256,7 +285,7 @@ __device__ __forceinline__ void inverse_ba(cx<R> (&w)[N1 / N2][N2], cx<R> (&v)[N
         const int s = t + N2 * r;
 #pragma unroll
         for (int q = 0; q < N2; ++q) {
-            const cx<R> z = (q == 0) ? w[r][q] : cmulc(w[r][q], twB[q * N1 + N2 * r]);
+            const cx<R> z = (q == 0) ? w[r][q] : tw_mulc(w[r][q], twB[q * N1 + N2 * r]);
             lds[Sh::slot(c, g, s, q)] = z;
         }
     }
@@ -268,7 +297,9 @@ __device__ __forceinline__ void inverse_ba(cx<R> (&w)[N1 / N2][N2], cx<R> (&v)[N
 
 // The round trip of one tile in registers: forward, phase multiply,
 // inverse, then the fused epilogue (nonlinear phase, norm partial sum).
-template <typename S, typename R, int N1, int N2, int TS, int G>
+// IMAG selects the imaginary-time form of the FP32 phase multiply and NL
+// factor at compile time (FP64 arithmetic reads kFlagImag at run time).
+template <typename S, typename R, int N1, int N2, int TS, int G, bool IMAG>
 __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1], const TileGeom& gm,
                                                 const SweepArgs& a, cx<R>* lds, const cx<R>* twA,
                                                 const cx<R>* twB, int c, int g, int t, double& norm_acc)
@@ -284,15 +315,36 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
     forward_ab<R, N1, N2, TS, G>(v, w, lds, twA, c, g, t);
     const cx<R>* __restrict__ ph =
         static_cast<const cx<R>*>(a.phase) + (long long)gm.row * a.phase_row_stride + t;
+    if constexpr (sizeof(R) == 4) {
+        // FP32: X m/N with X/N exact (N a power of two).  Real-time tables
+        // hold the lifting coefficients (tan(b/2), sin b) of half the phase,
+        // b = phi/2 with phi in (-pi, pi], applied twice: a complex multiply
+        // by rounded (cos phi, sin phi) loses 1 - cos(phi) of the low modes
+        // to rounding on every pass and drifts the norm.  Imaginary-time
+        // tables hold d = (m - 1)/N (real): X/N + X d.
+        constexpr float inv_n = 1.0f / float(N1 * N2);
 #pragma unroll
-    for (int r = 0; r < RB; ++r)
+        for (int r = 0; r < RB; ++r)
 #pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) w[r][k2] = cmul(w[r][k2], ph[N2 * r + N1 * k2]);
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const cx<R> d = ph[N2 * r + N1 * k2];
+                const cx<R> x = w[r][k2];
+                if constexpr (IMAG)
+                    w[r][k2] = {fmaf(x.re, d.re, x.re * inv_n), fmaf(x.im, d.re, x.im * inv_n)};
+                else
+                    w[r][k2] = lift(lift({x.re * inv_n, x.im * inv_n}, d.re, d.im), d.re, d.im);
+            }
+    } else {
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) w[r][k2] = cmul(w[r][k2], ph[N2 * r + N1 * k2]);
+    }
     inverse_ba<R, N1, N2, TS, G>(w, v, lds, twB, c, g, t);
     if (flags & kFlagNL) {
         if (a.save) store_line<S, R, N1>(v, static_cast<cx<S>*>(a.save), gm);
         const R beta = R(a.beta), dt = R(a.dt);
-        const bool imag = flags & kFlagImag;
+        const bool imag = sizeof(R) == 4 ? IMAG : (flags & kFlagImag) != 0;
 #pragma unroll
         for (int j = 0; j < N1; ++j) nl_phase<R>(v[j], vp[j], beta, dt, imag);
     }
@@ -335,11 +387,11 @@ __global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
     cx<R> v[N1];
     R vp[N1];
 
-    if constexpr (MODE == kRoundTrip) {
+    if constexpr (MODE == kRoundTrip || MODE == kRoundTripImag) {
         load_line<S, R, N1>(v, psi, gm);
         if (flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
         double nacc = 0.0;
-        round_trip_tile<S, R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
+        round_trip_tile<S, R, N1, N2, TS, G, MODE == kRoundTripImag>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
         if (flags & kFlagNorm) {
             double acc1[1] = {nacc};
             block_partials<R, Sh::NT, 1>(acc1, a.partial, nblocks, tile);
@@ -439,7 +491,7 @@ __global__ __launch_bounds__(TS * G * N2) void k_batch_sweep(const SweepArgs* __
     load_line<S, R, N1>(v, psi, gm);
     if (a.flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
     double nacc = 0.0;
-    round_trip_tile<S, R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
+    round_trip_tile<S, R, N1, N2, TS, G, true>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
     if (a.flags & kFlagNorm) {
         double acc1[1] = {nacc};
         block_partials<R, Sh::NT, 1>(acc1, a.partial, a.ntiles, tile);
@@ -451,7 +503,7 @@ __global__ __launch_bounds__(TS * G * N2) void k_batch_sweep(const SweepArgs* __
 // blockIdx.x, blockIdx.x + gridDim.x, ... and issues the loads of its next
 // tile before transforming the current one, so HBM traffic overlaps the
 // FFT arithmetic within every wavefront.
-template <typename S, typename R, int N1, int N2, int TS, int G>
+template <typename S, typename R, int N1, int N2, int TS, int G, bool IMAG>
 __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
 {
     using Sh = SweepShape<R, N1, N2, TS, G>;
@@ -491,7 +543,7 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
             load_line<S, R, N1>(vn, psi, gn);
             if (nl) load_pot<S, R, N1>(vpn, V, gn);
         }
-        round_trip_tile<S, R, N1, N2, TS, G>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
+        round_trip_tile<S, R, N1, N2, TS, G, IMAG>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
         store_line<S, R, N1>(v, psi, gm);
         if (!more) break;
         __syncthreads();  // every lane is done with the exchange buffer

@@ -120,8 +120,9 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
     static bool launch(int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a)
     {
         if (idx == 0) {
-            if constexpr (V0::P && MODE == kRoundTrip)
-                hipLaunchKernelGGL((k_sweep_pipe<S, R, V0::N1, V0::N2, V0::TS, V0::G>), grid, block, 0, s, a);
+            if constexpr (V0::P && (MODE == kRoundTrip || MODE == kRoundTripImag))
+                hipLaunchKernelGGL((k_sweep_pipe<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE == kRoundTripImag>),
+                                   grid, block, 0, s, a);
             else
                 hipLaunchKernelGGL((k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W>), grid, block, 0, s,
                                    a);
@@ -146,7 +147,8 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
             if constexpr (V0::P) {
                 int nb = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &nb, reinterpret_cast<const void*>(&k_sweep_pipe<S, R, V0::N1, V0::N2, V0::TS, V0::G>),
+                        &nb,
+                        reinterpret_cast<const void*>(&k_sweep_pipe<S, R, V0::N1, V0::N2, V0::TS, V0::G, false>),
                         V0::TS * V0::G * V0::N2, 0) != hipSuccess)
                     return 1;
                 return nb > 0 ? nb : 1;

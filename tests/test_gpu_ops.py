@@ -125,7 +125,9 @@ def test_error_paths(gpu):
 def test_nonlinear_phase_accuracy(gpu, prec):
     """solveNL! on ψ = 1 with V = θ/Δt gives exp(-iθ): the device sine/cosine
     (Cody-Waite reduction + Taylor series, library fallback for huge |θ|)
-    against numpy over |θ| up to 3e4, multiples of π/2 included."""
+    against numpy over |θ| up to 3e4, multiples of π/2 included.  FP32
+    rotates by two lifting steps of -θ/2 (six rounded fused multiply-adds of
+    unit size, worst case ~7 ulp, unbiased), hence 6e-7."""
     FT = np.float64 if prec == "f64" else np.float32
     rng = np.random.default_rng(0)
     th = np.concatenate([rng.uniform(-4, 4, 4096), rng.uniform(-200, 200, 4096), rng.uniform(-3e4, 3e4, 2048),
@@ -144,7 +146,7 @@ def test_nonlinear_phase_accuracy(gpu, prec):
     exact = np.exp(-1j * (th / dt).astype(FT).astype(np.float64) * dt)
     err = np.max(np.abs(got - exact))
     print(f"{prec}: max |error| {err:.3e}")
-    assert err < (4e-16 if prec == "f64" else 3e-7)
+    assert err < (4e-16 if prec == "f64" else 6e-7)
 
 
 def test_lap_rot_and_nl_compose_to_split2(gpu, orc):
