@@ -156,6 +156,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-launches", type=int, default=40)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the 1000-step FP32/F32X-vs-FP64 accuracy run at the stable dt")
     ap.add_argument("--traffic-driver", action="store_true",
                     help="only launch the kernels profiled by the rocprofv3 --pmc traffic runs")
     args = ap.parse_args(argv)
@@ -213,6 +215,21 @@ def main(argv=None):
     achieved = launch_bytes / (dom_ms * 1e-3)
     traffic = load_traffic("traffic_sweep_strided_f32_256.json")
 
+    # Accuracy at the largest stable step: at dt = 5e-4 the unfiltered split is
+    # past its stability limit (D |coeffD| kmax^2 dt = 12.3 > pi) and any two
+    # precisions, or FP64 runs one ulp apart, decorrelate within ~100 steps.
+    accuracy = None
+    if not args.no_accuracy:
+        from sfdhip.precision import run_study
+        st = run_study(n=n, steps=1000, dt=1e-4, checkpoints=1, with_rounded=False)
+        row = st["rows"][-1]
+        accuracy = {"dt": 1e-4, "steps": 1000, "vs": "FP64 GPU run, same ψ0",
+                    "f32_rel_l2": row["f32"]["rel_l2_vs_f64"], "f32x_rel_l2": row["f32x"]["rel_l2_vs_f64"],
+                    "f32_norm_drift": row["f32"]["norm_drift"], "f32_energy_drift": row["f32"]["energy_drift"],
+                    "f64_norm_drift": row["f64"]["norm_drift"], "f64_energy_drift": row["f64"]["energy_drift"]}
+        print(f"[bench] accuracy dt=1e-4 1000 steps: f32 relL2 {accuracy['f32_rel_l2']:.3e}, "
+              f"f32x relL2 {accuracy['f32x_rel_l2']:.3e}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         t_cpu, cores = cpu_leg(psi0, mp, args.cpu_steps)
@@ -251,6 +268,7 @@ def main(argv=None):
         "f32_vs_f64_rel_l2": err32,
         "f32x_vs_f64_rel_l2": err32x,
         "f32_vs_f64_steps": args.warmup + args.steps,
+        "accuracy_stable_dt": accuracy,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": "k_sweep strided (y/z round trip), FP32, 256³",

@@ -32,6 +32,9 @@ extern "C" {
 #define SFD_ERR_OOM (-4)   /* device allocation failed                     */
 #define SFD_ERR_HIP (-5)   /* a HIP runtime call or kernel launch failed   */
 
+/* SFD_F32 applies every rotation (DFT constants, twiddles, kinetic and
+ * nonlinear phases) as FP32 lifting steps, whose forward/inverse pairs carry
+ * no systematic rounding error. */
 #define SFD_F32 1  /* complex FP32 field and arithmetic, FP32 V, FP64 reductions */
 #define SFD_F64 2  /* complex FP64 field and arithmetic, FP64 V                  */
 #define SFD_F32X 3 /* complex FP32 field and FP32 V in memory, FP64 arithmetic   */

@@ -4,8 +4,10 @@ sweeps, phases) against an FP64 run of the same workload on the GPU.
 Three runs advance from one seeded initial state:
 
 * ``f64``  -- the FP64 reference trajectory;
-* ``f32``  -- the FP32 field with FP32 arithmetic (FP64 geometry; twiddles
-  and phase tables formed in FP64 and rounded once; FP64 reductions);
+* ``f32``  -- the FP32 field with FP32 arithmetic (FP64 geometry; every
+  rotation -- DFT constants, twiddles, kinetic and nonlinear phases -- as
+  lifting steps whose FP32 round trips carry no systematic error; FP64
+  reductions);
 * ``f32x`` -- the FP32 field with FP64 in-register arithmetic and tables
   (same HBM bytes as ``f32``);
 * ``f64r`` -- FP64 started from the initial state rounded to FP32, which
