@@ -160,6 +160,7 @@ struct sfd_ctx {
     int tuned = 0;
     int pipe_occ[3] = {0, 0, 0};  // resident workgroups per CU (persistent variants)
     int ncu = 1;
+    long long last_nblocks = 0;   // workgroups of the last sweep launched (its partial count)
     double* xi[3] = {nullptr, nullptr, nullptr};
     double* coord[3] = {nullptr, nullptr, nullptr};
     double* partial = nullptr;
@@ -302,7 +303,7 @@ struct Launch {
     long long nblocks;
 };
 
-Launch sweep_geometry(const sfd_ctx* c, int axis, SweepArgs& a, bool round_trip)
+Launch sweep_geometry(const sfd_ctx* c, int axis, SweepArgs& a, bool persistent)
 {
     const VariantInfo& v = c->vi[axis];
     const int NL = v.ts * v.g;
@@ -322,8 +323,8 @@ Launch sweep_geometry(const sfd_ctx* c, int axis, SweepArgs& a, bool round_trip)
         a.ntiles = (long long)a.ntx * others;
         L.grid = dim3((unsigned)a.ntx, (unsigned)others, 1);
     }
-    if (v.pipe && round_trip) {
-        // persistent: as many workgroups as stay resident
+    if (persistent) {
+        // pipelined variant: as many workgroups as stay resident
         const long long resident = (long long)c->pipe_occ[axis] * c->ncu;
         L.grid = dim3((unsigned)(a.ntiles < resident ? a.ntiles : resident), 1, 1);
     }
@@ -339,18 +340,26 @@ int launch_sweep(sfd_ctx* c, int axis, SweepArgs a)
     using S = typename P::S;
     (void)sizeof(R);
     (void)sizeof(S);
-    Launch L = sweep_geometry(c, axis, a, MODE == kRoundTrip || MODE == kRoundTripImag);
+    // The pipelined kernel of a pipelined variant takes round trips without a
+    // nonlinear epilogue or pending scale and with one phase row; other
+    // sweeps run k_sweep.
+    const bool persistent = c->vi[axis].pipe && (MODE == kRoundTrip || MODE == kRoundTripImag) &&
+                            !(a.flags & (kFlagNL | kFlagScale)) && a.phase_row_stride == 0;
+    Launch L = sweep_geometry(c, axis, a, persistent);
     a.tw = c->tw[axis];
-    if (!launch_variant<P, MODE>(c->n[axis], axis == 0, c->var[axis], L.grid, L.block, c->stream, a))
+    if (!launch_variant<P, MODE>(c->n[axis], axis == 0, c->var[axis], L.grid, L.block, c->stream, a, persistent))
         return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", c->var[axis], c->n[axis]);
     SFD_HIP(hipGetLastError());
+    c->last_nblocks = L.nblocks;
     return SFD_OK;
 }
 
-long long sweep_nblocks(const sfd_ctx* c, int axis, bool round_trip = true)
+// Workgroups of a non-persistent sweep: an upper bound on the partial sums
+// any sweep of the axis writes.
+long long sweep_nblocks(const sfd_ctx* c, int axis)
 {
     SweepArgs a{};
-    return sweep_geometry(c, axis, a, round_trip).nblocks;
+    return sweep_geometry(c, axis, a, false).nblocks;
 }
 
 // Select launch variant `idx` for `axis` and (re)build the twiddle table of
@@ -489,7 +498,7 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
             }
         }
         if (imag) {
-            const long long nb = sweep_nblocks(c, D - 1);
+            const long long nb = c->last_nblocks;  // partials of the norm sweep just launched
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nb, 1, c->scal + 1,
                                c->dvol(), c->scal);
             SFD_HIP(hipGetLastError());
@@ -522,7 +531,7 @@ int energy_impl(sfd_ctx* c, const void* psi, const void* V, double cD, double be
         a.flags = ax == 0 ? kFlagPointwise : 0;
         if (ax == 0 && c->ndim >= 2) { a.coord = c->coord[1]; a.coord_mod = c->n[1]; }
         if (ax == 1) a.coord = c->coord[0];
-        const long long nb = sweep_nblocks(c, ax, false);
+        const long long nb = sweep_nblocks(c, ax);
         SFD_TRY(ensure_partial(c, 5 * nb));
         a.partial = c->partial;
         SFD_TRY(launch_sweep<P, kEnergy>(c, ax, a));

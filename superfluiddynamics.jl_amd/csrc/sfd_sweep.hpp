@@ -132,6 +132,25 @@ __device__ __forceinline__ void block_partials(double (&acc)[K], double* partial
     }
 }
 
+// s_waitcnt vmcnt(N) (gfx9 encoding: vmcnt in bits 3:0 and 15:14, expcnt
+// and lgkmcnt left at their no-wait maxima).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Workgroup barrier that orders LDS only.  __syncthreads() also fences
+// global memory, i.e. waits for every outstanding global load, including a
+// tile prefetched for later; the sweeps exchange data through LDS alone.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Geometry of one workgroup tile: a uniform base (SGPRs) plus a 32-bit lane
 // offset; element j of the lane's line sits at base + lane + j*jstride.
 // Lanes past the end of the field read a clamped (valid) line and never
@@ -248,6 +267,26 @@ __device__ __forceinline__ cx<R> tw_mulc(cx<R> v, cx<R> w)
     else return cmulc(v, w);
 }
 
+// Phase-table row of a tile in global memory (rows differ when Omega != 0).
+template <typename R>
+__device__ __forceinline__ const cx<R>* row_phase(const SweepArgs& a, const TileGeom& gm)
+{
+    return static_cast<const cx<R>*>(a.phase) + (long long)gm.row * a.phase_row_stride;
+}
+
+// store_line without the lane predicate: lanes past the end of the field
+// hold a clamped copy of the last valid line, transformed by the same
+// operations, so they write bit-identical values to its addresses.  Keeps the
+// number of issued stores independent of the tile, which lets the compiler
+// wait for a prefetched tile without also waiting for these stores.
+template <typename S, typename R, int N1>
+__device__ __forceinline__ void store_line_all(const cx<R> (&v)[N1], cx<S>* psi, const TileGeom& g)
+{
+    cx<S>* b = psi + g.ubase;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) lane_ref(b, j * g.jstride, g.lane) = narrow<S>(v[j]);
+}
+
 // Forward transform of the tile's lines: phase A (N1-point DFTs, twiddles),
 // LDS exchange, phase B (N2-point DFTs).  On return w[r][k2] holds
 // X[s + N1 k2] of row s = t + N2 r.
@@ -261,7 +300,7 @@ __device__ __forceinline__ void forward_ab(cx<R> (&v)[N1], cx<R> (&w)[N1 / N2][N
     for (int s = 1; s < N1; ++s) v[s] = tw_mul(v[s], twA[s * N2]);
 #pragma unroll
     for (int s = 0; s < N1; ++s) lds[Sh::slot(c, g, s, t)] = v[s];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int r = 0; r < N1 / N2; ++r) {
 #pragma unroll
@@ -289,7 +328,7 @@ __device__ __forceinline__ void inverse_ba(cx<R> (&w)[N1 / N2][N2], cx<R> (&v)[N
             lds[Sh::slot(c, g, s, q)] = z;
         }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int s = 0; s < N1; ++s) v[s] = lds[Sh::slot(c, g, s, t)];
     dft<N1, +1, R>(v);
@@ -302,7 +341,8 @@ __device__ __forceinline__ void inverse_ba(cx<R> (&w)[N1 / N2][N2], cx<R> (&v)[N
 template <typename S, typename R, int N1, int N2, int TS, int G, bool IMAG>
 __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1], const TileGeom& gm,
                                                 const SweepArgs& a, cx<R>* lds, const cx<R>* twA,
-                                                const cx<R>* twB, int c, int g, int t, double& norm_acc)
+                                                const cx<R>* twB, const cx<R>* phase, int c, int g, int t,
+                                                double& norm_acc)
 {
     constexpr int RB = N1 / N2;
     const int flags = a.flags;
@@ -313,8 +353,7 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
     }
     cx<R> w[RB][N2];
     forward_ab<R, N1, N2, TS, G>(v, w, lds, twA, c, g, t);
-    const cx<R>* __restrict__ ph =
-        static_cast<const cx<R>*>(a.phase) + (long long)gm.row * a.phase_row_stride + t;
+    const cx<R>* __restrict__ ph = phase + t;
     if constexpr (sizeof(R) == 4) {
         // FP32: X m/N with X/N exact (N a power of two).  Real-time tables
         // hold the lifting coefficients (tan(b/2), sin b) of half the phase,
@@ -391,7 +430,8 @@ __global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
         load_line<S, R, N1>(v, psi, gm);
         if (flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
         double nacc = 0.0;
-        round_trip_tile<S, R, N1, N2, TS, G, MODE == kRoundTripImag>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
+        round_trip_tile<S, R, N1, N2, TS, G, MODE == kRoundTripImag>(v, vp, gm, a, lds, twA, twB,
+                                                                     row_phase<R>(a, gm), c, g, t, nacc);
         if (flags & kFlagNorm) {
             double acc1[1] = {nacc};
             block_partials<R, Sh::NT, 1>(acc1, a.partial, nblocks, tile);
@@ -491,7 +531,7 @@ __global__ __launch_bounds__(TS * G * N2) void k_batch_sweep(const SweepArgs* __
     load_line<S, R, N1>(v, psi, gm);
     if (a.flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
     double nacc = 0.0;
-    round_trip_tile<S, R, N1, N2, TS, G, true>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
+    round_trip_tile<S, R, N1, N2, TS, G, true>(v, vp, gm, a, lds, twA, twB, row_phase<R>(a, gm), c, g, t, nacc);
     if (a.flags & kFlagNorm) {
         double acc1[1] = {nacc};
         block_partials<R, Sh::NT, 1>(acc1, a.partial, a.ntiles, tile);
@@ -502,7 +542,11 @@ __global__ __launch_bounds__(TS * G * N2) void k_batch_sweep(const SweepArgs* __
 // Persistent, software-pipelined round trip: each workgroup walks tiles
 // blockIdx.x, blockIdx.x + gridDim.x, ... and issues the loads of its next
 // tile before transforming the current one, so HBM traffic overlaps the
-// FFT arithmetic within every wavefront.
+// FFT arithmetic within every wavefront.  The twiddle and phase tables are
+// staged in LDS first: vector-memory counters retire in order, so a table
+// load from memory inside the loop would wait for the prefetched tile too.
+// No nonlinear epilogue, no pending scale and a single phase row (Omega = 0
+// or the z axis): other sweeps run k_sweep.
 template <typename S, typename R, int N1, int N2, int TS, int G, bool IMAG>
 __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
 {
@@ -510,6 +554,7 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
     constexpr int N = Sh::N;
     constexpr int NL = Sh::NL;
     __shared__ cx<R> lds[Sh::LDS];
+    __shared__ cx<R> tabs[3 * N];  // twA | twB | phase
 
     const int tid = threadIdx.x;
     const int c = tid % TS;
@@ -517,42 +562,48 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
     const int g = tid / (TS * N2);
     const int L = c + TS * g;
     const int flags = a.flags;
-    const bool nl = flags & kFlagNL;
-    const cx<R>* __restrict__ twA = static_cast<const cx<R>*>(a.tw) + t;
-    const cx<R>* __restrict__ twB = static_cast<const cx<R>*>(a.tw) + N + t;
     cx<S>* __restrict__ psi = static_cast<cx<S>*>(a.psi);
-    const S* __restrict__ V = static_cast<const S*>(a.V);
-
     long long tile = blockIdx.x;
     double nacc = 0.0;
     cx<R> v[N1];
-    R vp[N1];
+    const R vp[N1] = {};
     TileGeom gm = tile_geom<N, N2, TS, NL>(a, tile, L, t);
-    if (tile < a.ntiles) {
-        load_line<S, R, N1>(v, psi, gm);
-        if (nl) load_pot<S, R, N1>(vp, V, gm);
+    if (tile < a.ntiles) load_line<S, R, N1>(v, psi, gm);
+    // stage the tables after the first tile's loads, so the loop starts with
+    // no tile load outstanding
+    {
+        // unrolled, branch-free when NT divides 3N: the compiler then knows
+        // every load above has completed once the tables are written
+        const cx<R>* gtw = static_cast<const cx<R>*>(a.tw);
+        const cx<R>* gph = static_cast<const cx<R>*>(a.phase);
+#pragma unroll
+        for (int k = 0; k < (3 * N + Sh::NT - 1) / Sh::NT; ++k) {
+            const int i = tid + k * Sh::NT;
+            if ((3 * N) % Sh::NT == 0 || i < 3 * N) tabs[i] = i < 2 * N ? gtw[i] : gph[i - 2 * N];
+        }
     }
+    const cx<R>* twA = tabs + t;
+    const cx<R>* twB = tabs + N + t;
+    const cx<R>* ph = tabs + 2 * N;
+    lds_barrier();  // tables staged
     while (tile < a.ntiles) {
         const long long next = tile + gridDim.x;
         const bool more = next < a.ntiles;
         TileGeom gn;
         cx<R> vn[N1];
-        R vpn[N1];
         if (more) {
             gn = tile_geom<N, N2, TS, NL>(a, next, L, t);
             load_line<S, R, N1>(vn, psi, gn);
-            if (nl) load_pot<S, R, N1>(vpn, V, gn);
         }
-        round_trip_tile<S, R, N1, N2, TS, G, IMAG>(v, vp, gm, a, lds, twA, twB, c, g, t, nacc);
-        store_line<S, R, N1>(v, psi, gm);
+        round_trip_tile<S, R, N1, N2, TS, G, IMAG>(v, vp, gm, a, lds, twA, twB, ph, c, g, t, nacc);
+        store_line_all<S, R, N1>(v, psi, gm);
         if (!more) break;
-        __syncthreads();  // every lane is done with the exchange buffer
+        // the next tile's loads were issued before these N1 stores: wait for
+        // them here, with the stores still in flight
+        wait_vmcnt<N1>();
+        lds_barrier();  // every lane is done with the exchange buffer
 #pragma unroll
         for (int j = 0; j < N1; ++j) v[j] = vn[j];
-        if (nl) {
-#pragma unroll
-            for (int j = 0; j < N1; ++j) vp[j] = vpn[j];
-        }
         gm = gn;
         tile = next;
     }

@@ -69,8 +69,8 @@ SFD_VARIANTS(PF32, 32, (VL<V<8, 4, 1, 64>>), (VL<V<8, 4, 16, 1>>))
 SFD_VARIANTS(PF32, 64, (VL<V<8, 8, 1, 32>, V<16, 4, 1, 64>>), (VL<V<8, 8, 16, 1>, V<8, 8, 32, 1>>))
 SFD_VARIANTS(PF32, 128, (VL<V<16, 8, 1, 32>, V<16, 8, 1, 16>, V<32, 4, 1, 32>, V<16, 8, 1, 16, 1>>),
              (VL<V<16, 8, 16, 1>, V<16, 8, 32, 1>, V<32, 4, 32, 1>, V<16, 8, 16, 1, 1>>))
-SFD_VARIANTS(PF32, 256, (VL<V<16, 16, 1, 16>, V<16, 16, 1, 8>, V<32, 8, 1, 16>, V<16, 16, 1, 16, 1>>),
-             (VL<V<16, 16, 16, 1>, V<16, 16, 32, 1>, V<32, 8, 16, 1>, V<16, 16, 8, 1, 1>>))
+SFD_VARIANTS(PF32, 256, (VL<V<16, 16, 1, 16>, V<16, 16, 1, 8>, V<32, 8, 1, 16>, V<16, 16, 1, 16, 1>, V<16, 16, 1, 8, 1>>),
+             (VL<V<16, 16, 16, 1>, V<16, 16, 32, 1>, V<32, 8, 16, 1>, V<16, 16, 16, 1, 1>, V<16, 16, 8, 1, 1>>))
 SFD_VARIANTS(PF32, 512, (VL<V<32, 16, 1, 8>, V<32, 16, 1, 4>>), (VL<V<32, 16, 16, 1>, V<32, 16, 8, 1>>))
 SFD_VARIANTS(PF32, 1024, (VL<V<32, 32, 1, 4>, V<32, 32, 1, 2>>), (VL<V<32, 32, 8, 1>, V<32, 32, 4, 1>>))
 
@@ -83,7 +83,8 @@ SFD_VARIANTS(PF64, 32, (VL<V<8, 4, 1, 64>>), (VL<V<8, 4, 16, 1>>))
 SFD_VARIANTS(PF64, 64, (VL<V<8, 8, 1, 32>, V<16, 4, 1, 32>>), (VL<V<8, 8, 16, 1>, V<8, 8, 32, 1>>))
 SFD_VARIANTS(PF64, 128, (VL<V<16, 8, 1, 16>, V<16, 8, 1, 8>, V<32, 4, 1, 16>>),
              (VL<V<16, 8, 16, 1>, V<16, 8, 32, 1>, V<32, 4, 16, 1>>))
-SFD_VARIANTS(PF64, 256, (VL<V<16, 16, 1, 8>, V<16, 16, 1, 4>>), (VL<V<16, 16, 16, 1>, V<16, 16, 8, 1>>))
+SFD_VARIANTS(PF64, 256, (VL<V<16, 16, 1, 8>, V<16, 16, 1, 4>, V<16, 16, 1, 8, 1>>),
+             (VL<V<16, 16, 16, 1>, V<16, 16, 8, 1>, V<16, 16, 8, 1, 1>>))
 SFD_VARIANTS(PF64, 512, (VL<V<32, 16, 1, 4>>), (VL<V<32, 16, 8, 1>, V<32, 16, 4, 1>>))
 SFD_VARIANTS(PF64, 1024, (VL<V<32, 32, 1, 2>>), (VL<V<32, 32, 4, 1>, V<32, 32, 2, 1>>))
 
@@ -100,7 +101,7 @@ struct Dispatch;
 template <typename P, int MODE>
 struct Dispatch<P, MODE, VL<>> {
     static bool info(int, VariantInfo*) { return false; }
-    static bool launch(int, dim3, dim3, hipStream_t, const SweepArgs&) { return false; }
+    static bool launch(int, dim3, dim3, hipStream_t, const SweepArgs&, bool) { return false; }
     static bool launch_batch(int, dim3, dim3, hipStream_t, const SweepArgs*, const int*) { return false; }
     static int occupancy(int) { return 0; }
 };
@@ -117,18 +118,22 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
         }
         return Dispatch<P, MODE, VL<Vs...>>::info(idx - 1, out);
     }
-    static bool launch(int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a)
+    // persistent: run the pipelined kernel of a pipelined variant (the
+    // caller sized the grid for it); otherwise its plain k_sweep shape
+    static bool launch(int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a, bool persistent)
     {
         if (idx == 0) {
-            if constexpr (V0::P && (MODE == kRoundTrip || MODE == kRoundTripImag))
-                hipLaunchKernelGGL((k_sweep_pipe<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE == kRoundTripImag>),
-                                   grid, block, 0, s, a);
-            else
-                hipLaunchKernelGGL((k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W>), grid, block, 0, s,
-                                   a);
+            if constexpr (V0::P && (MODE == kRoundTrip || MODE == kRoundTripImag)) {
+                if (persistent) {
+                    hipLaunchKernelGGL((k_sweep_pipe<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE == kRoundTripImag>),
+                                       grid, block, 0, s, a);
+                    return true;
+                }
+            }
+            hipLaunchKernelGGL((k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W>), grid, block, 0, s, a);
             return true;
         }
-        return Dispatch<P, MODE, VL<Vs...>>::launch(idx - 1, grid, block, s, a);
+        return Dispatch<P, MODE, VL<Vs...>>::launch(idx - 1, grid, block, s, a, persistent);
     }
     static bool launch_batch(int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs* descs,
                              const int* done)
@@ -167,10 +172,11 @@ int variant_count(bool contig)
 }
 
 template <typename P, int MODE, int N>
-bool launch_variant_n(bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a)
+bool launch_variant_n(bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a,
+                      bool persistent)
 {
-    if (contig) return Dispatch<P, MODE, typename Variants<P, N>::contig>::launch(idx, grid, block, s, a);
-    return Dispatch<P, MODE, typename Variants<P, N>::strided>::launch(idx, grid, block, s, a);
+    if (contig) return Dispatch<P, MODE, typename Variants<P, N>::contig>::launch(idx, grid, block, s, a, persistent);
+    return Dispatch<P, MODE, typename Variants<P, N>::strided>::launch(idx, grid, block, s, a, persistent);
 }
 
 template <typename P, int N>
@@ -238,10 +244,11 @@ bool launch_batch(int n, bool contig, int idx, dim3 grid, dim3 block, hipStream_
 }
 
 template <typename P, int MODE>
-bool launch_variant(int n, bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a)
+bool launch_variant(int n, bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a,
+                    bool persistent)
 {
 #define SFD_CASE(N_) \
-    if (n == N_) return launch_variant_n<P, MODE, N_>(contig, idx, grid, block, s, a);
+    if (n == N_) return launch_variant_n<P, MODE, N_>(contig, idx, grid, block, s, a, persistent);
     SFD_FOR_EACH_N(SFD_CASE)
 #undef SFD_CASE
     return false;
