@@ -100,6 +100,12 @@ def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, kernel_launches, autotune
     return ms_step, per_axis, state, nrm, variants, tune
 
 
+def sweep_kernel_name(v):
+    """Kernel of a variant (n1, n2, ts, g, pipelined, waves)."""
+    n1, n2, ts, g, pipe = v[:5]
+    return f"{'k_sweep_pipe' if pipe else 'k_sweep'}<float, float, {n1}, {n2}, {ts}, {g}>"
+
+
 def cpu_leg(psi0, mp, cpu_steps):
     """The FP64 CPU oracle (C + OpenMP restatement of NumModelSplit2) on the
     same 256³ workload: one untimed step, then cpu_steps timed steps
@@ -117,8 +123,9 @@ def cpu_leg(psi0, mp, cpu_steps):
 
 def traffic_driver(n=256, launches=5):
     """Launches for the rocprofv3 --pmc runs: `launches` round trips of each
-    axis of the 256³ FP32 workload (x: contiguous kernel, then y and z: the
-    strided kernel), then `launches` in-place passes of a plain elementwise
+    axis of the 256³ FP32 workload under the autotuned variants (x: the
+    contiguous kernel, then y and z: the strided kernel), then `launches`
+    in-place passes of a plain elementwise
     kernel over the same 128 MiB (torch mul_), the calibration whose
     algorithmic traffic is exactly one read and one write of the field."""
     import ctypes
@@ -129,6 +136,7 @@ def traffic_driver(n=256, launches=5):
     mp = metric_params(n)
     g = sfd.Grid((n, n, n), mp["bounds"], FT=np.float32)
     f = sfd.Field(g)
+    f.autotune()  # the variants the bench's FP32 leg runs
     f.set(random_phase_3d(n, seed=0))
     lib, ctx = sfd.load(), f.ctx()
     psi = ctypes.c_void_p(f.ϕ.data_ptr())
@@ -271,7 +279,7 @@ def main(argv=None):
         "accuracy_stable_dt": accuracy,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "k_sweep strided (y/z round trip), FP32, 256³",
+                     "kernel": sweep_kernel_name(res["f32"]["variants"][1]) + " strided (y/z round trip), FP32, 256³",
                      "bytes_per_launch": launch_bytes},
         "cpu_baseline": cpu,
     }
