@@ -145,6 +145,15 @@ int sfd_batch_imag_run(sfd_batch* b, void* const* psi, const void* const* V, con
                        double coeffD, double dt, int64_t max_steps, double tol, int check_every,
                        int use_graph, int64_t* steps_out, double* dpsi_out);
 
+/* Autotuner of a batch: for every size group and axis, times each compiled
+ * launch variant of the batched sweep (identity round trips on the batch's
+ * own scratch, the fields are not touched) and keeps the fastest, like
+ * sfd_autotune for one context.  times_ms (may be NULL) receives at least
+ * 16*ninst floats: [16 g + 8 axis + variant] = ms per launch, -1 where there
+ * is none, g in the batch's internal group order.  No reference counterpart
+ * (FFTW plan flags, Plans.jl:847-848, play this role there). */
+int sfd_batch_autotune(sfd_batch* b, float* times_ms);
+
 /* Adjoint of the real-time Strang split (no counterpart in the reference):
  * loss L = 1 - |<target|psi_N>|^2, <a|b> = sum conj(a) b dV, after nsteps
  * NumModelSplit2 steps from psi0 (psi0 is not modified).  Returns *loss and

@@ -794,6 +794,7 @@ struct sfd_batch {
         int n[2];
         int first = 0, count = 0;
         VariantInfo vi[2] = {};
+        int var[2] = {0, 0};  // launch variant per axis (sfd_batch_autotune)
         void* tw[2] = {nullptr, nullptr};
         long long ntiles[2] = {0, 0};
     };
@@ -823,6 +824,48 @@ struct sfd_batch {
 };
 
 namespace {
+
+// Launch variant `idx` for one axis of a size group: its twiddles (rebuilt
+// when the line split changes) and tile count.
+template <typename P>
+int batch_group_variant(sfd_batch* b, sfd_batch::Group& G, int ax, int idx)
+{
+    using R = typename P::R;
+    VariantInfo vi;
+    if (!variant_info<P>(G.n[ax], ax == 0, idx, &vi))
+        return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", idx, G.n[ax]);
+    if (!G.tw[ax] || G.vi[ax].n1 != vi.n1) {
+        std::vector<cx<R>> w;
+        fill_twiddles<R>(w, G.n[ax], vi.n1, vi.n2);
+        if (G.tw[ax]) {
+            SFD_HIP(hipStreamSynchronize(b->stream));
+            SFD_HIP(hipFree(G.tw[ax]));
+            G.tw[ax] = nullptr;
+        }
+        SFD_HIP(hipMalloc(&G.tw[ax], w.size() * sizeof(cx<R>)));
+        SFD_TRY(upload_sync(G.tw[ax], w.data(), w.size() * sizeof(cx<R>)));
+    }
+    G.vi[ax] = vi;
+    G.var[ax] = idx;
+    const int NL = vi.ts * vi.g;
+    G.ntiles[ax] = ax == 0 ? (G.n[1] + NL - 1) / NL : (G.n[0] + NL - 1) / NL;
+    return SFD_OK;
+}
+
+// Line geometry of instance `in` along `ax` under its group's variant.
+void batch_geom(SweepArgs& a, const sfd_batch::Inst& in, const sfd_batch::Group& G, int ax)
+{
+    if (ax == 0) {
+        a.S = 1;
+        a.nlines = in.n[1];
+    } else {
+        a.S = in.n[0];
+        a.OS = (long long)in.n[0] * in.n[1];
+        a.ncols = in.n[0];
+        a.ntx = (int)G.ntiles[1];
+    }
+    a.ntiles = G.ntiles[ax];
+}
 
 template <typename P>
 int batch_setup(sfd_batch* b)
@@ -854,17 +897,18 @@ int batch_setup(sfd_batch* b)
     const int ni = b->ninst;
     long long parts = 0, sites = 0;
     for (auto& G : b->groups) {
-        for (int ax = 0; ax < 2; ++ax) {
-            if (!variant_info<P>(G.n[ax], ax == 0, 0, &G.vi[ax]))
-                return fail(SFD_ERR_SHAPE, "no sweep variant for line length %d", G.n[ax]);
-            std::vector<cx<R>> w;
-            fill_twiddles<R>(w, G.n[ax], G.vi[ax].n1, G.vi[ax].n2);
-            SFD_HIP(hipMalloc(&G.tw[ax], w.size() * sizeof(cx<R>)));
-            SFD_TRY(upload_sync(G.tw[ax], w.data(), w.size() * sizeof(cx<R>)));
-            const int NL = G.vi[ax].ts * G.vi[ax].g;
-            G.ntiles[ax] = ax == 0 ? (G.n[1] + NL - 1) / NL : (G.n[0] + NL - 1) / NL;
+        for (int ax = 0; ax < 2; ++ax) SFD_TRY(batch_group_variant<P>(b, G, ax, 0));
+        // norm partials come from the y sweep: room for its variant with the
+        // most tiles, so the autotuner may switch variants
+        long long most = 0;
+        for (int v = 0; v < n_variants<P>(G.n[1], false); ++v) {
+            VariantInfo vi;
+            if (variant_info<P>(G.n[1], false, v, &vi)) {
+                const int NL = vi.ts * vi.g;
+                most = std::max(most, (long long)((G.n[0] + NL - 1) / NL));
+            }
         }
-        parts += G.ntiles[1] * G.count;
+        parts += most * G.count;
     }
     for (auto& in : b->inst) sites += in.nsite;
     SFD_HIP(hipMalloc(&b->d_inst, ni * sizeof(BatchInstDev)));
@@ -925,16 +969,7 @@ int batch_bind(sfd_batch* b, void* const* psi, const void* const* V, const doubl
             a.scale = b->d_scale + i;
             a.beta = beta[in.user];
             a.dt = dt;
-            if (ax == 0) {
-                a.S = 1;
-                a.nlines = in.n[1];
-            } else {
-                a.S = in.n[0];
-                a.OS = (long long)in.n[0] * in.n[1];
-                a.ncols = in.n[0];
-                a.ntx = (int)G.ntiles[1];
-            }
-            a.ntiles = G.ntiles[ax];
+            batch_geom(a, in, G, ax);
             // x (rescale), y (+ nonlinear), x, y (+ norm partials)
             a.flags = kFlagImag;
             if (k == 0) a.flags |= kFlagScale;
@@ -973,7 +1008,7 @@ int batch_enqueue(sfd_batch* b, int k, bool check, double tol)
                 const VariantInfo& vi = G.vi[ax];
                 dim3 grid((unsigned)G.ntiles[ax], 1, (unsigned)G.count);
                 dim3 block(vi.ts * vi.g * vi.n2);
-                if (!launch_batch<P>(G.n[ax], ax == 0, 0, grid, block, b->stream,
+                if (!launch_batch<P>(G.n[ax], ax == 0, G.var[ax], grid, block, b->stream,
                                      b->d_desc + (size_t)kind * ni + G.first, b->d_done + G.first))
                     return fail(SFD_ERR_SHAPE, "batch: no kernel for line length %d", G.n[ax]);
                 SFD_HIP(hipGetLastError());
@@ -992,6 +1027,104 @@ int batch_enqueue(sfd_batch* b, int k, bool check, double tol)
         }
     }
     return SFD_OK;
+}
+
+void batch_drop_graph(sfd_batch* b);
+
+// Batch autotuner: for every size group and axis with more than one compiled
+// variant, time the identity round trip (tau = 0) of k_batch_sweep over the
+// group's instances on the zeroed snapshot buffer (the fields are not
+// touched), 2 warm-up launches then the best of 3 batches of 8, and keep the
+// fastest.  times[16 g + 8 ax + v] = ms per launch of variant v (-1 if none).
+template <typename P>
+int batch_autotune_impl(sfd_batch* b, float* times)
+{
+    using R = typename P::R;
+    using S = typename P::S;
+    const int ni = b->ninst;
+    if (times)
+        for (int i = 0; i < 16 * ni; ++i) times[i] = -1.0f;
+    batch_drop_graph(b);
+    long long sites = 0;
+    for (auto& in : b->inst) sites += in.nsite;
+    SFD_HIP(hipEventRecord(b->ev, b->user_stream));
+    SFD_HIP(hipStreamWaitEvent(b->stream, b->ev, 0));
+    SFD_HIP(hipMemsetAsync(b->d_snap, 0, (size_t)sites * sizeof(cx<S>), b->stream));
+    SFD_HIP(hipMemsetAsync(b->d_done, 0, ni * sizeof(int), b->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<void*> tmp;  // identity phase tables
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(b->stream);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        for (void* p : tmp) (void)hipFree(p);
+    };
+    auto run = [&]() -> int {
+        SFD_HIP(hipEventCreate(&e0));
+        SFD_HIP(hipEventCreate(&e1));
+        for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+            auto& G = b->groups[gi];
+            for (int ax = 0; ax < 2; ++ax) {
+                const int nv = n_variants<P>(G.n[ax], ax == 0);
+                if (nv <= 1) continue;
+                const int n = G.n[ax];
+                std::vector<cx<R>> h(n, decay_entry<R>(0.0, 1.0 / (double)n));
+                void* ph = nullptr;
+                SFD_HIP(hipMalloc(&ph, n * sizeof(cx<R>)));
+                tmp.push_back(ph);
+                SFD_TRY(upload_sync(ph, h.data(), n * sizeof(cx<R>)));
+                float best = 1e30f;
+                int best_v = 0;
+                for (int v = 0; v < nv && v < 8; ++v) {
+                    SFD_TRY(batch_group_variant<P>(b, G, ax, v));
+                    std::vector<SweepArgs> desc(G.count);
+                    long long soff = 0;
+                    for (int i = 0; i < G.first; ++i) soff += b->inst[i].nsite;
+                    for (int k = 0; k < G.count; ++k) {
+                        const auto& in = b->inst[G.first + k];
+                        SweepArgs& a = desc[k];
+                        std::memset(&a, 0, sizeof a);
+                        a.psi = static_cast<char*>(b->d_snap) + soff * sizeof(cx<S>);
+                        a.phase = ph;
+                        a.tw = G.tw[ax];
+                        a.flags = kFlagImag;
+                        batch_geom(a, in, G, ax);
+                        soff += in.nsite;
+                    }
+                    SFD_TRY(upload_sync(b->d_desc, desc.data(), desc.size() * sizeof(SweepArgs)));
+                    const VariantInfo& vi = G.vi[ax];
+                    dim3 grid((unsigned)G.ntiles[ax], 1, (unsigned)G.count);
+                    dim3 block(vi.ts * vi.g * vi.n2);
+                    auto launch = [&]() {
+                        return launch_batch<P>(n, ax == 0, v, grid, block, b->stream, b->d_desc, b->d_done);
+                    };
+                    for (int w = 0; w < 2; ++w)
+                        if (!launch()) return fail(SFD_ERR_SHAPE, "batch autotune: no kernel %d for n = %d", v, n);
+                    SFD_HIP(hipGetLastError());
+                    float t_best = 1e30f;
+                    for (int rep = 0; rep < 3; ++rep) {
+                        SFD_HIP(hipEventRecord(e0, b->stream));
+                        for (int k = 0; k < 8; ++k) launch();
+                        SFD_HIP(hipEventRecord(e1, b->stream));
+                        SFD_HIP(hipEventSynchronize(e1));
+                        float ms = 0.0f;
+                        SFD_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        t_best = std::min(t_best, ms / 8.0f);
+                    }
+                    if (times) times[16 * gi + 8 * ax + v] = t_best;
+                    if (t_best < best) {
+                        best = t_best;
+                        best_v = v;
+                    }
+                }
+                SFD_TRY(batch_group_variant<P>(b, G, ax, best_v));
+            }
+        }
+        return SFD_OK;
+    };
+    const int rc = run();
+    cleanup();
+    return rc;
 }
 
 void batch_drop_graph(sfd_batch* b)
@@ -1312,6 +1445,14 @@ int sfd_batch_create(sfd_batch** out, int ninst, const int64_t* n, const double*
     }
     *out = b;
     return SFD_OK;
+}
+
+int sfd_batch_autotune(sfd_batch* b, float* times_ms)
+{
+    g_last_error.clear();
+    if (!b) return fail(SFD_ERR_NULL, "sfd_batch_autotune: null batch");
+    SFD_HIP(hipSetDevice(b->device));
+    return SFD_BY_PREC(b, batch_autotune_impl, b, times_ms);
 }
 
 int sfd_batch_destroy(sfd_batch* b)

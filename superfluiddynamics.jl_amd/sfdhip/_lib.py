@@ -34,8 +34,8 @@ EXPORTS = (
     "sfd_lap_rot", "sfd_lap_rot_axis", "sfd_solve_nl", "sfd_norm2", "sfd_normalize", "sfd_energy", "sfd_fft_axis", "sfd_alloc", "sfd_free",
     "sfd_upload", "sfd_download", "sfd_synchronize", "sfd_last_error", "sfd_version",
     "sfd_autotune", "sfd_ctx_get_variant", "sfd_ctx_set_variant",
-    "sfd_batch_create", "sfd_batch_destroy", "sfd_batch_imag_run", "sfd_adjoint", "sfd_project",
-    "sfd_inner",
+    "sfd_batch_create", "sfd_batch_destroy", "sfd_batch_imag_run", "sfd_batch_autotune", "sfd_adjoint",
+    "sfd_project", "sfd_inner",
 )
 
 
@@ -101,6 +101,7 @@ def load():
         "sfd_ctx_set_variant": ([vp, c_int, c_int], c_int),
         "sfd_batch_create": ([ctypes.POINTER(vp), c_int, i64p, dp, c_int, vp], c_int),
         "sfd_batch_destroy": ([vp], c_int),
+        "sfd_batch_autotune": ([vp, ctypes.POINTER(ctypes.c_float)], c_int),
         "sfd_adjoint": ([vp, vp, vp, vp, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_i64, vp, dp, dp, vp], c_int),
         "sfd_project": ([vp, vp, vp, c_int, dp], c_int),
         "sfd_inner": ([vp, vp, vp, dp], c_int),

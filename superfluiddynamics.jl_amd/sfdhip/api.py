@@ -504,6 +504,20 @@ class BatchImagSolver:
                                      ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
         self.h = h
 
+    def autotune(self):
+        """Pick the fastest batched-sweep variant per size group and axis
+        (sfd_batch_autotune; the fields are not touched).  Returns a list of
+        {axis: [ms per launch of each variant]} per size group."""
+        ni = len(self.fields)
+        times = (ctypes.c_float * (16 * ni))()
+        L.check(self.lib.sfd_batch_autotune(self.h, times))
+        out = []
+        for g in range(ni):
+            row = {a: [t for t in times[16 * g + 8 * a:16 * g + 8 * a + 8] if t >= 0] for a in range(2)}
+            if any(row.values()):
+                out.append(row)
+        return out
+
     def run(self, max_steps=20000, tol=1e-9, check_every=100, use_graph=True):
         """Returns (steps taken per field, last ||Δψ|| per field)."""
         ni = len(self.fields)

@@ -19,7 +19,7 @@ import numpy as np
 
 
 def run(prec="f32", max_steps=20000, dt=5e-3, tol=1e-9, check_every=100, use_graph=True, seed=0,
-        counts=((128, 32), (256, 48), (512, 16))):
+        counts=((128, 32), (256, 48), (512, 16)), autotune=True):
     import torch
     from . import api as sfd
     from .workloads import batch_ground_states
@@ -34,6 +34,7 @@ def run(prec="f32", max_steps=20000, dt=5e-3, tol=1e-9, check_every=100, use_gra
         params.append(sfd.GrossPitaevskiiParameters(pot=sfd.PotentialArray(f, pr["V"]), coeffΔ=-0.5, β=pr["beta"]))
         fields.append(f)
     solver = sfd.BatchImagSolver(fields, params, dt)
+    tune = solver.autotune() if autotune else None
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     steps, dpsi = solver.run(max_steps=max_steps, tol=tol, check_every=check_every, use_graph=use_graph)
@@ -51,7 +52,7 @@ def run(prec="f32", max_steps=20000, dt=5e-3, tol=1e-9, check_every=100, use_gra
         "precision": prec, "dt": dt, "tol": tol, "max_steps": max_steps, "check_every": check_every,
         "graph": use_graph, "wall_s": wall, "site_updates_per_s": site_steps / wall,
         "converged": int(np.sum(steps < max_steps)), "steps_min": int(steps.min()), "steps_max": int(steps.max()),
-        "dpsi_median": float(np.median(dpsi)), "sample": sample,
+        "dpsi_median": float(np.median(dpsi)), "sample": sample, "autotune_ms": tune,
     }
 
 
@@ -63,8 +64,10 @@ def main(argv=None):
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--check-every", type=int, default=100)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args(argv)
-    res = run(args.prec, args.steps, args.dt, args.tol, args.check_every, not args.no_graph)
+    res = run(args.prec, args.steps, args.dt, args.tol, args.check_every, not args.no_graph,
+              autotune=not args.no_autotune)
     print(json.dumps(res))
     return 0
 

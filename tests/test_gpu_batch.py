@@ -45,6 +45,26 @@ def test_batch_equals_single_field_runs(gpu, prec, graph):
         assert gpu.norm(f) == pytest.approx(1.0, rel=1e-6)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_batch_autotune_keeps_results(gpu, prec):
+    """After sfd_batch_autotune picks a variant per size group and axis, the
+    batch still ends where the single-field runs end, and the fields the
+    tuner ran beside were not touched."""
+    fields, params, inits = _make(gpu, prec, seed=7)
+    solver = gpu.BatchImagSolver(fields, params, 4e-3)
+    before = [f.to_numpy() for f in fields]
+    tune = solver.autotune()
+    # axes with one compiled variant are not timed
+    assert tune and any(len(v) > 1 for row in tune for v in row.values())
+    for f, b in zip(fields, before):
+        assert np.array_equal(f.to_numpy(), b)
+    steps, _ = solver.run(max_steps=23, tol=0.0, check_every=10, use_graph=True)
+    assert list(steps) == [23] * len(fields)
+    for f, (psi0, V, bounds, beta) in zip(fields, inits):
+        ref = gpu_run(gpu, psi0, bounds, V=V, beta=beta, dt=4e-3, nsteps=23, order=2, imag=True, prec=prec)
+        assert rel_l2(f.to_numpy(), ref) <= (1e-14 if prec == "f64" else 1e-6)
+
+
 def test_batch_matches_oracle(gpu, orc):
     fields, params, inits = _make(gpu, "f64", seed=3)
     gpu.BatchImagSolver(fields, params, 5e-3).run(max_steps=20, tol=0.0, check_every=7)
