@@ -804,6 +804,11 @@ struct sfd_batch {
     hipStream_t user_stream = nullptr;
     hipStream_t stream = nullptr;  // own stream (graph capture needs a non-default stream)
     hipEvent_t ev = nullptr;
+    // one stream per size group: the groups' chains of launches are
+    // independent, so they fork from `stream` and join it again each chunk
+    std::vector<hipStream_t> gstream;
+    std::vector<hipEvent_t> gdone;
+    hipEvent_t fork = nullptr;
     std::vector<Inst> inst;        // internal (group-major) order
     std::vector<Group> groups;
     BatchInstDev* d_inst = nullptr;
@@ -922,6 +927,15 @@ int batch_setup(sfd_batch* b)
     SFD_HIP(hipMalloc(&b->d_snap, (size_t)sites * sizeof(cx<S>)));
     SFD_HIP(hipMalloc(&b->d_desc, 4 * (size_t)ni * sizeof(SweepArgs)));
     b->phase.assign(2 * (size_t)ni, nullptr);
+    SFD_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        SFD_HIP(hipStreamCreate(&st));
+        b->gstream.push_back(st);
+        SFD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        b->gdone.push_back(ev);
+    }
     return SFD_OK;
 }
 
@@ -989,42 +1003,53 @@ int batch_bind(sfd_batch* b, void* const* psi, const void* const* V, const doubl
 }
 
 // Enqueue k imaginary-time Strang steps of every unfinished instance; the
-// last step is a convergence check when `check` is set.
+// last step is a convergence check when `check` is set.  Each size group
+// runs its k steps on its own stream (forked from and joined back to the
+// batch stream), so the groups' small launches overlap.
 template <typename P>
 int batch_enqueue(sfd_batch* b, int k, bool check, double tol)
 {
     using S = typename P::S;
     const int ni = b->ninst;
-    for (int s = 0; s < k; ++s) {
-        const bool chk = check && s == k - 1;
-        if (chk) {
-            hipLaunchKernelGGL(k_batch_snapshot<S>, dim3(sfd_batch::kDiffBlocks, ni), dim3(256), 0, b->stream,
-                               b->d_inst, b->d_scale, b->d_snap_scale, b->d_done);
-            SFD_HIP(hipGetLastError());
-        }
-        for (int kind = 0; kind < 4; ++kind) {
-            const int ax = kind % 2;
-            for (auto& G : b->groups) {
+    constexpr int KB = sfd_batch::kDiffBlocks;
+    SFD_HIP(hipEventRecord(b->fork, b->stream));
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        const auto& G = b->groups[gi];
+        hipStream_t st = b->gstream[gi];
+        SFD_HIP(hipStreamWaitEvent(st, b->fork, 0));
+        const int f = G.first;
+        for (int s = 0; s < k; ++s) {
+            const bool chk = check && s == k - 1;
+            if (chk) {
+                hipLaunchKernelGGL(k_batch_snapshot<S>, dim3(KB, G.count), dim3(256), 0, st, b->d_inst + f,
+                                   b->d_scale + f, b->d_snap_scale + f, b->d_done + f);
+                SFD_HIP(hipGetLastError());
+            }
+            for (int kind = 0; kind < 4; ++kind) {
+                const int ax = kind % 2;
                 const VariantInfo& vi = G.vi[ax];
                 dim3 grid((unsigned)G.ntiles[ax], 1, (unsigned)G.count);
                 dim3 block(vi.ts * vi.g * vi.n2);
-                if (!launch_batch<P>(G.n[ax], ax == 0, G.var[ax], grid, block, b->stream,
-                                     b->d_desc + (size_t)kind * ni + G.first, b->d_done + G.first))
+                if (!launch_batch<P>(G.n[ax], ax == 0, G.var[ax], grid, block, st,
+                                     b->d_desc + (size_t)kind * ni + f, b->d_done + f))
                     return fail(SFD_ERR_SHAPE, "batch: no kernel for line length %d", G.n[ax]);
                 SFD_HIP(hipGetLastError());
             }
-        }
-        hipLaunchKernelGGL(k_batch_finalize, dim3(ni), dim3(256), 0, b->stream, b->d_inst, b->d_scale, b->d_done,
-                           b->d_steps);
-        SFD_HIP(hipGetLastError());
-        if (chk) {
-            hipLaunchKernelGGL(k_batch_diff<S>, dim3(sfd_batch::kDiffBlocks, ni), dim3(256), 0, b->stream,
-                               b->d_inst, b->d_scale, b->d_snap_scale, b->d_done, b->d_dpart);
+            hipLaunchKernelGGL(k_batch_finalize, dim3(G.count), dim3(256), 0, st, b->d_inst + f, b->d_scale + f,
+                               b->d_done + f, b->d_steps + f);
             SFD_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_batch_diff_finalize, dim3(ni), dim3(256), 0, b->stream, b->d_dpart,
-                               sfd_batch::kDiffBlocks, b->d_inst, b->d_dpsi, b->d_done, tol);
-            SFD_HIP(hipGetLastError());
+            if (chk) {
+                hipLaunchKernelGGL(k_batch_diff<S>, dim3(KB, G.count), dim3(256), 0, st, b->d_inst + f,
+                                   b->d_scale + f, b->d_snap_scale + f, b->d_done + f, b->d_dpart + (size_t)f * KB);
+                SFD_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_batch_diff_finalize, dim3(G.count), dim3(256), 0, st,
+                                   b->d_dpart + (size_t)f * KB, KB, b->d_inst + f, b->d_dpsi + f, b->d_done + f,
+                                   tol);
+                SFD_HIP(hipGetLastError());
+            }
         }
+        SFD_HIP(hipEventRecord(b->gdone[gi], st));
+        SFD_HIP(hipStreamWaitEvent(b->stream, b->gdone[gi], 0));
     }
     return SFD_OK;
 }
@@ -1212,6 +1237,9 @@ void batch_release(sfd_batch* b)
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->ev) (void)hipEventDestroy(b->ev);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (hipEvent_t e : b->gdone) (void)hipEventDestroy(e);
+    for (hipStream_t st : b->gstream) (void)hipStreamDestroy(st);
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
