@@ -276,6 +276,9 @@ def main(argv=None):
         "f32_vs_f64_rel_l2": err32,
         "f32x_vs_f64_rel_l2": err32x,
         "f32_vs_f64_steps": args.warmup + args.steps,
+        "f32_vs_f64_note": "at the metric dt the unfiltered split is past its stability limit "
+                           "(D |coeffD| kmax^2 dt = 12.3 > pi): any two precisions, or FP64 runs one rounding "
+                           "apart, decorrelate to ~sqrt(2); accuracy_stable_dt holds the comparison at dt = 1e-4",
         "accuracy_stable_dt": accuracy,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,

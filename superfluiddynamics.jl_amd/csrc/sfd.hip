@@ -875,7 +875,6 @@ void batch_geom(SweepArgs& a, const sfd_batch::Inst& in, const sfd_batch::Group&
 template <typename P>
 int batch_setup(sfd_batch* b)
 {
-    using R = typename P::R;
     using S = typename P::S;
     // groups of equal extents, internal order group-major
     std::vector<sfd_batch::Inst> sorted;
