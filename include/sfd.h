@@ -109,8 +109,10 @@ int sfd_fft_axis(sfd_ctx* ctx, const void* in, void* out, int axis, int directio
  * FFTW/CUFFT planning, Plans.jl:847-848,991-993): for each axis, time every
  * compiled launch variant of the round-trip sweep (line split N = n1*n2,
  * workgroup shape) on a scratch field and keep the fastest for this context.
- * scratch: a device buffer of nsites complex values, or NULL to allocate one
- * temporarily.  times_ms (optional, 24 floats) receives the per-launch time
+ * scratch: a device buffer of nsites complex values (overwritten with
+ * unit-modulus pseudo-random data, since sweep timings depend on the data),
+ * or NULL to allocate one temporarily.  times_ms (optional, 24 floats)
+ * receives the per-launch time
  * of variant v of axis a at [a*8 + v] (-1 where none).  Synchronises. */
 int sfd_autotune(sfd_ctx* ctx, void* scratch, float* times_ms);
 
