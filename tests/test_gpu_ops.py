@@ -106,6 +106,28 @@ def test_full_size_invariants(gpu, prec):
     assert back < (1e-12 if prec == "f64" else 2e-5)
 
 
+@pytest.mark.parametrize("autotune", [False, True], ids=["default-variants", "autotuned"])
+def test_metric_workload_matches_oracle(gpu, orc, autotune):
+    """The 256³ metric workload itself (V = 0, β = 1/ξ², Δt = 5e-4,
+    random-phase ψ0): 5 FP64 Strang steps on the GPU against the oracle,
+    with the default launch variants and with the autotuned (pipelined)
+    ones, relative L2 within 1e-12 -- the FP64 bar, on the configuration the
+    speed metric is quoted on."""
+    from sfdhip.workloads import metric_params, random_phase_3d
+    mp = metric_params(256)
+    psi0 = random_phase_3d(256, seed=0)
+    f = _field(gpu, psi0, mp["bounds"], np.float64)
+    if autotune:
+        f.autotune()
+        f.set(psi0)
+    p = gpu.GrossPitaevskiiParameters(pot=gpu.PotentialZero(f), coeffΔ=-0.5, β=mp["beta"])
+    gpu.NumModelSplit2(f, p, mp["dt"], 5, 5).steps(5)
+    ref = orc.split_step(psi0.copy(), mp["bounds"], coeffD=-0.5, beta=mp["beta"], dt=mp["dt"], nsteps=5, order=2)
+    err = rel_l2(f.to_numpy(), ref)
+    print(f"256³ FP64 5 steps vs oracle ({'autotuned' if autotune else 'default'}): {err:.3e}")
+    assert err < 1e-12
+
+
 def test_error_paths(gpu):
     g = gpu.Grid((32, 32), ((-1, 1), (-1, 1)))
     f = gpu.Field(g)
