@@ -230,7 +230,10 @@ class Field:
 
     def set(self, values):
         """Copy a host array (numpy, x-fastest shape) into ϕ."""
-        self.data[0].copy_(torch.as_tensor(np.ascontiguousarray(values)).to(self.data[0].dtype))
+        arr = np.ascontiguousarray(values)
+        if not arr.flags.writeable:  # torch wants writable buffers (broadcast views are not)
+            arr = arr.copy()
+        self.data[0].copy_(torch.as_tensor(arr).to(self.data[0].dtype))
 
     def to_numpy(self):
         return self.data[0].detach().cpu().numpy()
@@ -550,12 +553,7 @@ def inner(f: Field, a, b) -> complex:
     return complex(out[0], out[1])
 
 
-def adjoint_gradient(n: _SplitBase, target, nsteps: int, checkpoint_every: int = 0, want_state: bool = False):
-    """Reverse-mode gradient of L = 1 - |<target|ψ(T)>|² through ``nsteps``
-    real-time Strang steps of model ``n`` from its current field (which is
-    left unchanged).  Returns (L, dL/dV as an FP64 device tensor, dL/dβ[, ψ(T)]).
-    Forward states are recomputed from √N checkpoints (or every
-    ``checkpoint_every`` steps)."""
+def _adjoint_call(n: _SplitBase, Vptr, beta: float, target, nsteps: int, checkpoint_every: int, want_state: bool):
     if n.mode != L.SFD_REAL_TIME or n.order != 2:
         raise L.ArgError(L.SFD_ERR_ARG, "the adjoint is defined for the real-time Strang split")
     f = n.f
@@ -565,11 +563,58 @@ def adjoint_gradient(n: _SplitBase, target, nsteps: int, checkpoint_every: int =
     gV = torch.zeros(f.g.shape, dtype=torch.float64, device="cuda")
     gb, loss = ctypes.c_double(), ctypes.c_double()
     out = torch.empty_like(f.ϕ) if want_state else None
-    L.check(c.lib.sfd_adjoint(c.h, ctypes.c_void_p(f.ϕ.data_ptr()), _vptr(p), ctypes.c_void_p(tgt.data_ptr()),
-                              p.coeffΔ, p.β, p.Ω, n.Δt, int(nsteps), int(checkpoint_every),
+    L.check(c.lib.sfd_adjoint(c.h, ctypes.c_void_p(f.ϕ.data_ptr()), Vptr, ctypes.c_void_p(tgt.data_ptr()),
+                              p.coeffΔ, float(beta), p.Ω, n.Δt, int(nsteps), int(checkpoint_every),
                               ctypes.c_void_p(gV.data_ptr()), ctypes.byref(gb), ctypes.byref(loss),
                               ctypes.c_void_p(out.data_ptr()) if want_state else None))
-    res = (loss.value, gV, gb.value)
+    return loss.value, gV, gb.value, out
+
+
+class FidelityLoss(torch.autograd.Function):
+    """L(V, β) = 1 - |<target|ψ(T)>|² after ``nsteps`` real-time Strang steps of
+    model ``n`` from its current field, as a torch autograd function (the
+    Python counterpart of a ChainRulesCore ``rrule``).  ``forward`` runs
+    ``sfd_adjoint``, which forms dL/dV and dL/dβ along with the loss (√N
+    checkpoint/recompute), and ``backward`` scales them by the incoming
+    gradient.  ``V`` is a real tensor of the grid's shape, used in the field's
+    precision; ``beta`` a 0-d tensor.  The model's own potential and β are not
+    used."""
+
+    @staticmethod
+    def forward(ctx, V, beta, n, target, nsteps, checkpoint_every=0):
+        f = n.f
+        rdt = torch.float32 if f.g.FT is np.float32 else torch.float64
+        if tuple(V.shape) != tuple(f.g.shape):
+            raise L.ShapeError(L.SFD_ERR_SHAPE, f"V has shape {tuple(V.shape)}, the grid {tuple(f.g.shape)}")
+        Vd = V.detach().to(device="cuda", dtype=rdt).contiguous()
+        loss, gV, gb, _ = _adjoint_call(n, ctypes.c_void_p(Vd.data_ptr()), float(beta), target, nsteps,
+                                        checkpoint_every, False)
+        ctx.save_for_backward(gV, torch.tensor(gb, dtype=torch.float64, device=gV.device))
+        ctx.dtypes = (V.dtype, beta.dtype if torch.is_tensor(beta) else torch.float64)
+        return torch.tensor(loss, dtype=torch.float64, device=gV.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        gV, gb = ctx.saved_tensors
+        dV, db = ctx.dtypes
+        return (g * gV).to(dV), (g * gb).to(db), None, None, None, None
+
+
+def fidelity_loss(n: _SplitBase, V, beta, target, nsteps: int, checkpoint_every: int = 0):
+    """``FidelityLoss.apply``: a differentiable L(V, β) for torch autograd."""
+    if not torch.is_tensor(beta):
+        beta = torch.tensor(float(beta), dtype=torch.float64, device="cuda")
+    return FidelityLoss.apply(V, beta, n, target, nsteps, checkpoint_every)
+
+
+def adjoint_gradient(n: _SplitBase, target, nsteps: int, checkpoint_every: int = 0, want_state: bool = False):
+    """Reverse-mode gradient of L = 1 - |<target|ψ(T)>|² through ``nsteps``
+    real-time Strang steps of model ``n`` from its current field (which is
+    left unchanged).  Returns (L, dL/dV as an FP64 device tensor, dL/dβ[, ψ(T)]).
+    Forward states are recomputed from √N checkpoints (or every
+    ``checkpoint_every`` steps)."""
+    loss, gV, gb, out = _adjoint_call(n, _vptr(n.param), n.param.β, target, nsteps, checkpoint_every, want_state)
+    res = (loss, gV, gb)
     return res + (out,) if want_state else res
 
 

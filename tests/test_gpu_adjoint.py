@@ -139,3 +139,47 @@ def test_reduced_precision_adjoint_tracks_fp64(gpu, orc, prec):
     assert abs(l - l64) / abs(l64) < tol
     assert np.sqrt(np.sum((g - g64) ** 2) / np.sum(g64 ** 2)) < 10 * tol
     assert abs(b - b64) / abs(b64) < 10 * tol
+
+
+def test_autograd_function_matches_adjoint_and_finite_differences(gpu, orc):
+    """fidelity_loss (a torch.autograd.Function over sfd_adjoint):
+    loss.backward() gives the adjoint's dL/dV and dL/dβ, and torch's chain
+    rule through V(a) = V0 + Σ_j a_j B_j gives dL/da_j, which central
+    differences (h = 1e-4) confirm to 1e-6 relative."""
+    shape, bounds = (64, 32), ((-4.0, 4.0), (-3.0, 3.0))
+    psi0 = smooth_state(shape, bounds, seed=2)
+    psi0 = psi0 / orc.norm(psi0, bounds)
+    tgt = smooth_state(shape, bounds, seed=3)
+    tgt = tgt / orc.norm(tgt, bounds)
+    V0 = random_potential(shape, bounds, amp=3.0)
+    f, m = _model(gpu, psi0, bounds, V0, 9.0)
+    t = torch.as_tensor(tgt).to("cuda")
+    l_ref, gV_ref, gb_ref = gpu.adjoint_gradient(m, t, 30)
+
+    V = torch.as_tensor(V0).to("cuda").requires_grad_(True)
+    beta = torch.tensor(9.0, dtype=torch.float64, device="cuda", requires_grad=True)
+    loss = gpu.fidelity_loss(m, V, beta, t, 30)
+    loss.backward()
+    assert loss.item() == l_ref
+    assert torch.equal(V.grad, gV_ref)
+    assert beta.grad.item() == gb_ref
+
+    x = torch.linspace(bounds[0][0], bounds[0][1], shape[0] + 1, dtype=torch.float64, device="cuda")[:-1]
+    y = torch.linspace(bounds[1][0], bounds[1][1], shape[1] + 1, dtype=torch.float64, device="cuda")[:-1]
+    Y, X = torch.meshgrid(y, x, indexing="ij")
+    B = torch.stack([torch.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / 0.5) for cx, cy in ((0.5, 0.0), (-1.0, 0.5), (0.0, -1.0))])
+    V0d = torch.as_tensor(V0).to("cuda")
+    a = torch.zeros(3, dtype=torch.float64, device="cuda", requires_grad=True)
+    loss2 = gpu.fidelity_loss(m, V0d + torch.tensordot(a, B, dims=1), 9.0, t, 30)
+    loss2.backward()
+    h = 1e-4
+    with torch.no_grad():
+        for j in range(3):
+            e = torch.zeros(3, dtype=torch.float64, device="cuda")
+            e[j] = h
+            lp = gpu.fidelity_loss(m, V0d + torch.tensordot(a + e, B, dims=1), 9.0, t, 30).item()
+            lm = gpu.fidelity_loss(m, V0d + torch.tensordot(a - e, B, dims=1), 9.0, t, 30).item()
+            fd = (lp - lm) / (2 * h)
+            print(f"a_{j}: autograd {a.grad[j].item():+.10e} fd {fd:+.10e}")
+            assert abs(a.grad[j].item() - fd) <= 1e-6 * abs(fd)
+    assert rel_l2(f.to_numpy(), psi0) < 1e-15  # the model's field is left unchanged
