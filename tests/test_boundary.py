@@ -64,6 +64,14 @@ def test_null_arguments():
     assert lib.sfd_norm2(None, None, None) == -1
     assert lib.sfd_energy(None, None, None, -0.5, 1.0, 0.0, None) == -1
     assert lib.sfd_fft_axis(None, None, None, 0, -1) == -1
+    assert lib.sfd_autotune(None, None, None) == -1
+    assert lib.sfd_adjoint(None, None, None, None, -0.5, 1.0, 0.0, 1e-3, 1, 0, None, None, None, None) == -1
+    assert lib.sfd_inner(None, None, None, None) == -1
+    assert lib.sfd_project(None, None, None, 1, None) == -1
+    assert lib.sfd_batch_create(None, 1, None, None, 2, None) == -1
+    assert lib.sfd_batch_imag_run(None, None, None, None, -0.5, 1e-3, 1, 0.0, 1, 0, None, None) == -1
+    assert lib.sfd_batch_autotune(None, None) == -1
+    assert lib.sfd_batch_destroy(None) in (0, -1)
     assert lib.sfd_ctx_destroy(None) == 0
     assert b"null" in lib.sfd_last_error()
 
