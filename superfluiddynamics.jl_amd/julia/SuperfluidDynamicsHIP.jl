@@ -9,18 +9,27 @@
 #   ...   NumModelSplit2(field, param, Δt, niter, freqbckp) / timeStep! / solve! as usual
 #
 # Julia is not installed in the build container, so this file is not exercised
-# by the test suite; the same entry points are exercised through ctypes.
+# by the test suite; the same entry points are exercised through ctypes
+# (superfluiddynamics.jl_amd/sfdhip, tests/).  Dependencies of the drop-in:
+# SuperfluidDynamics, AMDGPU, ChainRulesCore.
 module SuperfluidDynamicsHIP
 
 using SuperfluidDynamics
 using SuperfluidDynamics: AbstractField, Field, Grid, GrossPitaevskiiParameters,
                           NumModelSplit1, NumModelSplit2, PotentialZero
 using AMDGPU: ROCArray, stream
+import ChainRulesCore
 
 const libsfd = normpath(joinpath(@__DIR__, "..", "sfdhip", "libsfdhip.so"))
 
-const SFD_F32, SFD_F64 = Cint(1), Cint(2)
+const SFD_F32, SFD_F64, SFD_F32X = Cint(1), Cint(2), Cint(3)
 const SFD_REAL_TIME, SFD_IMAG_TIME = Cint(0), Cint(1)
+const SFD_FORWARD, SFD_BACKWARD = Cint(-1), Cint(1)
+
+# Arithmetic of Float32 fields: SFD_F32 (FP32, lifting rotations, FP64
+# reductions) or SFD_F32X (FP32 storage, FP64 arithmetic).  Set before the
+# first step of a field.
+const FLOAT32_MODE = Ref(SFD_F32)
 
 last_error() = unsafe_string(ccall((:sfd_last_error, libsfd), Cstring, ()))
 
@@ -44,7 +53,7 @@ function SfdContext(g::Grid{N,FT}) where {N,FT}
     n = Int64[g.n..., ntuple(_ -> 1, 3 - N)...]
     lo = Cdouble[g.min..., ntuple(_ -> 0.0, 3 - N)...]
     hi = Cdouble[g.max..., ntuple(_ -> 1.0, 3 - N)...]
-    prec = FT === Float32 ? SFD_F32 : SFD_F64
+    prec = FT === Float32 ? FLOAT32_MODE[] : SFD_F64
     check(ccall((:sfd_ctx_create, libsfd), Cint,
                 (Ptr{Ptr{Cvoid}}, Cint, Ptr{Int64}, Ptr{Cdouble}, Ptr{Cdouble}, Cint, Ptr{Cvoid}),
                 h, N, n, lo, hi, prec, stream().stream))
@@ -73,6 +82,101 @@ end
 # timeStep! of splitting.jl:78-82 and 149-154 for fields in HIP memory.
 SuperfluidDynamics.timeStep!(n::NumModelSplit2{<:HipField}) = split!(n, 2, SFD_REAL_TIME)
 SuperfluidDynamics.timeStep!(n::NumModelSplit1{<:HipField}) = split!(n, 1, SFD_REAL_TIME)
+
+# solveLapRot! / solveNL! of splitting.jl:156-209.
+function SuperfluidDynamics.solveLapRot!(n::Union{NumModelSplit1{<:HipField},NumModelSplit2{<:HipField}}, Δt)
+    p = n.param
+    check(ccall((:sfd_lap_rot, libsfd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cint),
+                context(n.f).handle, devptr(n.f.ϕ), p.coeffΔ, p.Ω, Δt, SFD_REAL_TIME))
+end
+function SuperfluidDynamics.solveNL!(n::Union{NumModelSplit1{<:HipField},NumModelSplit2{<:HipField}}, Δt)
+    p = n.param
+    check(ccall((:sfd_solve_nl, libsfd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cint),
+                context(n.f).handle, devptr(n.f.ϕ), vptr(p), p.β, Δt, SFD_REAL_TIME))
+end
+
+# Directional transforms of Plans.jl:1132-1250 (mul_d! forward, ldiv_d!
+# normalised inverse), in the pen_x layout: the AbstractFFTs plug for other
+# FFT-based models.
+for (d, ax) in ((:x, 0), (:y, 1), (:z, 2))
+    mul = Symbol(:hip_mul_, d, :!)
+    ldv = Symbol(:hip_ldiv_, d, :!)
+    @eval $mul(out::HipField, f::HipField) =
+        check(ccall((:sfd_fft_axis, libsfd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint),
+                    context(f).handle, devptr(f.ϕ), devptr(out.ϕ), $ax, SFD_FORWARD))
+    @eval $ldv(out::HipField, f::HipField) =
+        check(ccall((:sfd_fft_axis, libsfd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint),
+                    context(f).handle, devptr(f.ϕ), devptr(out.ϕ), $ax, SFD_BACKWARD))
+end
+
+# Imaginary-time Strang steps with FP64 renormalisation after every step.
+imag_steps!(n::NumModelSplit2{<:HipField}, nsteps::Integer) = split!(n, 2, SFD_IMAG_TIME, nsteps)
+
+# Autotuner (like FFTW planning): fastest launch variant per axis, timed on
+# scratch data; the field is not touched.
+function autotune!(f::HipField)
+    times = fill(-1f0, 24)
+    check(ccall((:sfd_autotune, libsfd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cfloat}),
+                context(f).handle, C_NULL, times))
+    return reshape(times, 8, 3)
+end
+
+# Batched 2D imaginary-time ground states of uneven sizes (sfd_batch_*):
+# advances every field until ||ψ_{n+1} - ψ_n|| < tol or max_steps.
+function batch_ground_states!(fields::Vector{<:HipField}, params::Vector{<:GrossPitaevskiiParameters}, Δt;
+                              max_steps=20000, tol=1e-9, check_every=100, use_graph=true, tune=true)
+    ni = length(fields)
+    n = Int64[v for f in fields for v in f.g.n]
+    box = Cdouble[v for f in fields for a in 1:2 for v in (f.g.min[a], f.g.max[a])]
+    FT = eltype(real(first(fields).ϕ))
+    prec = FT === Float32 ? FLOAT32_MODE[] : SFD_F64
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:sfd_batch_create, libsfd), Cint, (Ptr{Ptr{Cvoid}}, Cint, Ptr{Int64}, Ptr{Cdouble}, Cint, Ptr{Cvoid}),
+                h, ni, n, box, prec, stream().stream))
+    try
+        tune && check(ccall((:sfd_batch_autotune, libsfd), Cint, (Ptr{Cvoid}, Ptr{Cfloat}), h[], C_NULL))
+        psi = Ptr{Cvoid}[devptr(f.ϕ) for f in fields]
+        V = Ptr{Cvoid}[vptr(p) for p in params]
+        β = Cdouble[p.β for p in params]
+        steps = zeros(Int64, ni)
+        dpsi = zeros(Cdouble, ni)
+        check(ccall((:sfd_batch_imag_run, libsfd), Cint,
+                    (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Cdouble}, Cdouble, Cdouble, Int64, Cdouble,
+                     Cint, Cint, Ptr{Int64}, Ptr{Cdouble}),
+                    h[], psi, V, β, first(params).coeffΔ, Δt, max_steps, tol, check_every, use_graph, steps, dpsi))
+        return steps, dpsi
+    finally
+        ccall((:sfd_batch_destroy, libsfd), Cint, (Ptr{Cvoid},), h[])
+    end
+end
+
+# Final-state fidelity L(V, β) = 1 - |<target|ψ(T)>|² after nsteps Strang
+# steps of n from its current field, with the reverse-mode rule below: the
+# adjoint split step (sfd_adjoint, √N checkpoint/recompute) forms dL/dV and
+# dL/dβ together with L.
+function adjoint_call(n::NumModelSplit2{<:HipField}, V, β, target::HipField, nsteps::Integer, every::Integer=0)
+    p = n.param
+    gV = ROCArray{Float64}(undef, size(n.f.ϕ))
+    gβ, L = Ref{Cdouble}(0), Ref{Cdouble}(0)
+    Vp = V === nothing ? C_NULL : reinterpret(Ptr{Cvoid}, pointer(V))
+    check(ccall((:sfd_adjoint, libsfd), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Int64, Int64,
+                 Ptr{Cvoid}, Ref{Cdouble}, Ref{Cdouble}, Ptr{Cvoid}),
+                context(n.f).handle, devptr(n.f.ϕ), Vp, devptr(target.ϕ), p.coeffΔ, β, p.Ω, n.Δt, nsteps, every,
+                reinterpret(Ptr{Cvoid}, pointer(gV)), gβ, L, C_NULL))
+    return L[], gV, gβ[]
+end
+
+fidelity_loss(n::NumModelSplit2{<:HipField}, V, β, target::HipField, nsteps::Integer) =
+    first(adjoint_call(n, V, β, target, nsteps))
+
+function ChainRulesCore.rrule(::typeof(fidelity_loss), n::NumModelSplit2{<:HipField}, V, β, target::HipField,
+                              nsteps::Integer)
+    L, gV, gβ = adjoint_call(n, V, β, target, nsteps)
+    pullback(ȳ) = (ChainRulesCore.NoTangent(), ChainRulesCore.NoTangent(), ȳ .* gV, ȳ * gβ,
+                   ChainRulesCore.NoTangent(), ChainRulesCore.NoTangent())
+    return L, pullback
+end
 
 # norm / normalize! of Fields.jl:172-200 with an FP64 reduction.
 function SuperfluidDynamics.norm(f::HipField)
