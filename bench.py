@@ -73,6 +73,7 @@ def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, kernel_launches, autotune
     ms_step = e0.elapsed_time(e1) / steps
     state = f.to_numpy()
     nrm = sfd.norm(f)
+    solve = solve_loop(sfd, torch, model, min(steps, 50))
     # Average duration of one launch of each sweep kernel, on the stream the
     # kernels run on (torch's current stream is the context's stream).
     lib = sfd.load()
@@ -97,7 +98,25 @@ def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, kernel_launches, autotune
         per_axis.append(sorted(reps)[1])
     del model, f
     torch.cuda.empty_cache()
-    return ms_step, per_axis, state, nrm, variants, tune
+    return ms_step, per_axis, state, nrm, variants, tune, solve
+
+
+def solve_loop(sfd, torch, model, k):
+    """Wall time of one solve! iteration (energy, then a Split2 step): the
+    energy as its own call before each step (sfd_energy + sfd_split_step),
+    and fused into the steps' first-half sweeps (sfd_split_step_energy)."""
+    sfd.steps_with_energy(model, 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        sfd.energy_all(model)
+        model.steps(1)
+    torch.cuda.synchronize()
+    sep = (time.perf_counter() - t0) / k
+    t0 = time.perf_counter()
+    sfd.steps_with_energy(model, k)
+    fused = (time.perf_counter() - t0) / k
+    return {"iterations": k, "separate_ms_per_iter": sep * 1e3, "fused_ms_per_iter": fused * 1e3}
 
 
 def sweep_kernel_name(v):
@@ -119,6 +138,28 @@ def cpu_leg(psi0, mp, cpu_steps):
     oracle.split_step(psi, mp["bounds"], nsteps=cpu_steps, **kw)
     t = (time.perf_counter() - t0) / cpu_steps
     return t, oracle.num_threads()
+
+
+def solve_driver(n=256, iters=20, prec="f32"):
+    """Launches for a rocprofv3 kernel trace of solve!'s loop on the
+    workload: `iters` iterations with the energy as its own call
+    (sfd_energy + sfd_split_step), then `iters` fused iterations
+    (sfd_split_step_energy)."""
+    import numpy as np
+    import torch
+    import sfdhip as sfd
+    from sfdhip.workloads import metric_params, random_phase_3d
+    mp = metric_params(n)
+    FT, compute = PRECISIONS[prec]
+    g = sfd.Grid((n, n, n), mp["bounds"], FT=np.dtype(FT), compute=compute)
+    f = sfd.Field(g)
+    f.autotune()
+    f.set(random_phase_3d(n, seed=0))
+    p = sfd.GrossPitaevskiiParameters(pot=sfd.PotentialZero(f), coeffΔ=mp["coeffD"], β=mp["beta"])
+    model = sfd.NumModelSplit2(f, p, mp["dt"], iters, iters)
+    r = solve_loop(sfd, torch, model, iters)
+    print(json.dumps({"solve_driver": prec, "n": n, **r}), flush=True)
+    return 0
 
 
 def traffic_driver(n=256, launches=5):
@@ -168,9 +209,13 @@ def main(argv=None):
                     help="skip the 1000-step FP32/F32X-vs-FP64 accuracy run at the stable dt")
     ap.add_argument("--traffic-driver", action="store_true",
                     help="only launch the kernels profiled by the rocprofv3 --pmc traffic runs")
+    ap.add_argument("--solve-driver", choices=["f32", "f32x", "f64"],
+                    help="only run solve!'s loop (separate and fused energies) for a kernel trace")
     args = ap.parse_args(argv)
     if args.traffic_driver:
         return traffic_driver(args.n)
+    if args.solve_driver:
+        return solve_driver(args.n, prec=args.solve_driver)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the whole {args.n}³ domain is resident on a single MI355X; "
               f"--gpus {args.gpus} is not supported (the slab-decomposed multi-GPU path is not built).",
@@ -191,9 +236,12 @@ def main(argv=None):
     res = {}
     states = {}
     for prec in ("f32", "f32x", "f64"):
-        ms, per_axis, st, nrm, variants, tune = gpu_leg(sfd, torch, psi0, mp, prec, args.steps, args.warmup,
-                                                         args.kernel_launches, not args.no_autotune)
-        res[prec] = dict(ms=ms, per_axis=per_axis, norm=nrm, variants=variants, tune=tune)
+        ms, per_axis, st, nrm, variants, tune, solve = gpu_leg(sfd, torch, psi0, mp, prec, args.steps,
+                                                                args.warmup, args.kernel_launches,
+                                                                not args.no_autotune)
+        res[prec] = dict(ms=ms, per_axis=per_axis, norm=nrm, variants=variants, tune=tune, solve=solve)
+        print(f"[bench] {prec}: solve! iteration {solve['separate_ms_per_iter']:.4f} ms with a separate energy, "
+              f"{solve['fused_ms_per_iter']:.4f} ms fused", file=sys.stderr, flush=True)
         print(f"[bench] {prec}: variants (n1,n2,ts,g) per axis {variants}; autotune ms {tune}",
               file=sys.stderr, flush=True)
         states[prec] = st
@@ -280,6 +328,11 @@ def main(argv=None):
                            "(D |coeffD| kmax^2 dt = 12.3 > pi): any two precisions, or FP64 runs one rounding "
                            "apart, decorrelate to ~sqrt(2); accuracy_stable_dt holds the comparison at dt = 1e-4",
         "accuracy_stable_dt": accuracy,
+        "solve_loop": {"note": "wall time of one solve! iteration (energy of the state, then one Split2 "
+                               "step) at the workload size: separate = sfd_energy + sfd_split_step per "
+                               "iteration; fused = sfd_split_step_energy (energy sums taken in the step's "
+                               "first-half sweeps)",
+                       **{p_: res[p_]["solve"] for p_ in ("f32", "f32x", "f64")}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": sweep_kernel_name(res["f32"]["variants"][1]) + " strided (y/z round trip), FP32, 256³",

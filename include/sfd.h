@@ -99,6 +99,20 @@ int sfd_normalize(sfd_ctx* ctx, void* psi, double* out_norm);
 int sfd_energy(sfd_ctx* ctx, const void* psi, const void* V, double coeffD, double beta,
                double Omega, double* out5);
 
+/* The loop of solve! (NumModels.jl:32-37) without its writers: nsteps
+ * iterations of (energy(n), timeStep!(n)).  energies (host, 5*nsteps
+ * doubles) receives at [5*s .. 5*s+4] the (EOmega, EDelta, Ebeta, E, mu) of
+ * sfd_energy for the state before step s; the steps are those of
+ * sfd_split_step.  In real time with Omega = 0 the sums come out of each
+ * step's first-half sweeps, so the energy costs no extra pass over psi (the
+ * y and z sums are taken after the x flow, which leaves them unchanged to
+ * round-off); with rotation or in imaginary time (where the flows damp)
+ * each step is preceded by sfd_energy.
+ * Synchronises the stream (every 4096 steps and at return). */
+int sfd_split_step_energy(sfd_ctx* ctx, void* psi, const void* V, double coeffD, double beta,
+                          double Omega, double dt, int64_t nsteps, int order, int mode,
+                          double* energies);
+
 /* One directional transform (mul_x!/mul_y!/mul_z! with direction
  * SFD_FORWARD, ldiv_x!/ldiv_y!/ldiv_z! with SFD_BACKWARD; Plans.jl:1132-1250).
  * axis 0 = x, 1 = y, 2 = z.  Input and output are both in pen_x layout

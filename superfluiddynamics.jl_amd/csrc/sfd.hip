@@ -68,25 +68,71 @@ __global__ void k_norm2(const cx<S>* __restrict__ psi, long long n, double* part
     block_partials<S, 256, 1>(acc, partial, gridDim.x, blockIdx.x);
 }
 
-// out[k] = sum_b partial[k*nb + b] in a fixed order (deterministic).
-// If scale_out != null: *scale_out = 1 / sqrt(out[0] * dv).
-__global__ void k_finalize(const double* __restrict__ partial, long long nb, int K, double* out,
-                           double dv, double* scale_out)
+// out[k] = sum_b partial[k*nb + b] for k < K, in a fixed order
+// (deterministic: the order depends only on nb and the workgroup size).
+// Each lane adds a strided subset with four independent accumulators, so its
+// loads are in flight together; waves then combine through LDS.  Runs on
+// one workgroup of kFinalizeThreads.
+constexpr int kFinalizeThreads = 1024;
+
+__device__ __forceinline__ void finalize_rows(const double* __restrict__ partial, long long nb, int K, double* out)
 {
-    __shared__ double red[256];
+    __shared__ double red[kFinalizeThreads / 64];
+    const int tid = threadIdx.x;
+    const long long bs = blockDim.x;
     for (int k = 0; k < K; ++k) {
-        double s = 0.0;
-        for (long long b = threadIdx.x; b < nb; b += blockDim.x) s += partial[k * nb + b];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
+        const double* __restrict__ p = partial + k * nb;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        long long b = tid;
+        for (; b + 3 * bs < nb; b += 4 * bs) {
+            s0 += p[b];
+            s1 += p[b + bs];
+            s2 += p[b + 2 * bs];
+            s3 += p[b + 3 * bs];
         }
-        if (threadIdx.x == 0) out[k] = red[0];
+        for (; b < nb; b += bs) s0 += p[b];
+        const double s = wave_sum((s0 + s1) + (s2 + s3));
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+            for (int w = 0; w < (int)(bs + 63) / 64; ++w) t += red[w];
+            out[k] = t;
+        }
         __syncthreads();
     }
+}
+
+// finalize_rows; if scale_out != null also *scale_out = 1 / sqrt(out[0] * dv).
+__global__ __launch_bounds__(kFinalizeThreads) void k_finalize(const double* __restrict__ partial, long long nb,
+                                                              int K, double* out, double dv, double* scale_out)
+{
+    finalize_rows(partial, nb, K, out);
     if (scale_out && threadIdx.x == 0) scale_out[0] = 1.0 / sqrt(out[0] * dv);
+}
+
+// Up to three independent finalize_rows in one launch (the per-axis energy
+// sums of a step): one workgroup per row, rows of set 0 first.  Launch
+// with sum K[i] workgroups.
+struct FinalizeSets {
+    const double* partial[3];
+    long long nb[3];
+    int K[3];
+    double* out[3];
+    int rows() const { return K[0] + K[1] + K[2]; }
+};
+
+__global__ __launch_bounds__(kFinalizeThreads) void k_finalize_sets(const FinalizeSets f)
+{
+    int i = 0, k = blockIdx.x;
+    while (k >= f.K[i]) k -= f.K[i++];
+    finalize_rows(f.partial[i] + k * f.nb[i], f.nb[i], 1, f.out[i] + k);
+}
+
+void launch_finalize(hipStream_t s, const double* partial, long long nb, int K, double* out, double dv = 0.0,
+                     double* scale_out = nullptr)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, s, partial, nb, K, out, dv, scale_out);
 }
 
 // psi *= s (divide = 0) or psi /= s (divide = 1) in FP64, rounded once to
@@ -184,6 +230,10 @@ struct sfd_ctx {
     double* coord[3] = {nullptr, nullptr, nullptr};
     double* partial = nullptr;
     long long partial_cap = 0;  // doubles
+    double* epartial = nullptr;  // energy round trips: per-workgroup sums
+    long long epartial_cap = 0;
+    double* ehist = nullptr;     // fused step + energy: 8 FP64 sums per step
+    long long ehist_cap = 0;
     double* scal = nullptr;     // [0] pending scale, [1..31] sums
     std::vector<PhaseEntry> phases;
 
@@ -301,19 +351,22 @@ int build_phase(sfd_ctx* c, int axis, double tau, double cD, double Om, int imag
     return SFD_OK;
 }
 
-int ensure_partial(sfd_ctx* c, long long need)
+// Grows a device workspace of the context to `need` doubles (contents lost).
+int ensure_buf(sfd_ctx* c, double** buf, long long* cap, long long need)
 {
-    if (need <= c->partial_cap) return SFD_OK;
-    if (c->partial) {
+    if (need <= *cap) return SFD_OK;
+    if (*buf) {
         SFD_HIP(hipStreamSynchronize(c->stream));
-        SFD_HIP(hipFree(c->partial));
-        c->partial = nullptr;
-        c->partial_cap = 0;
+        SFD_HIP(hipFree(*buf));
+        *buf = nullptr;
+        *cap = 0;
     }
-    SFD_HIP(hipMalloc(&c->partial, (size_t)need * sizeof(double)));
-    c->partial_cap = need;
+    SFD_HIP(hipMalloc(buf, (size_t)need * sizeof(double)));
+    *cap = need;
     return SFD_OK;
 }
+
+int ensure_partial(sfd_ctx* c, long long need) { return ensure_buf(c, &c->partial, &c->partial_cap, need); }
 
 // Launch geometry of one sweep under the axis's current variant: returns
 // grid/block and fills the line fields of the argument block.
@@ -360,8 +413,8 @@ int launch_sweep(sfd_ctx* c, int axis, SweepArgs a)
     (void)sizeof(R);
     (void)sizeof(S);
     // The pipelined kernel of a pipelined variant takes round trips without a
-    // nonlinear epilogue or pending scale and with one phase row; other
-    // sweeps run k_sweep.
+    // nonlinear epilogue, pending scale or energy sums and with one phase
+    // row; other sweeps run k_sweep.
     const bool persistent = c->vi[axis].pipe && (MODE == kRoundTrip || MODE == kRoundTripImag) &&
                             !(a.flags & (kFlagNL | kFlagScale)) && a.phase_row_stride == 0;
     Launch L = sweep_geometry(c, axis, a, persistent);
@@ -428,8 +481,12 @@ int base_args(sfd_ctx* c, int axis, SweepArgs& a)
 
 // One linear round trip along `axis` over tau, with optional fused flags.
 template <typename P>
+// One round trip of an axis.  With `epartial` (room for 4 * sweep_nblocks
+// doubles), the sweep also writes the energy sums of the state it loads
+// there, 4 per workgroup (kRoundTripEnergy; kFlagPointwise in `flags` adds
+// the pointwise sums).
 int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, double cD, double Om, int imag,
-               int flags, double beta, double dt, void* save = nullptr)
+               int flags, double beta, double dt, void* save = nullptr, double* epartial = nullptr)
 {
     using R = typename P::R;
     using S = typename P::S;
@@ -454,6 +511,13 @@ int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, doubl
     if (flags & kFlagNorm) {
         SFD_TRY(ensure_partial(c, 5 * sweep_nblocks(c, axis)));
         a.partial = c->partial;
+    }
+    if (epartial) {
+        a.epartial = epartial;
+        a.xi = c->xi[axis];
+        if constexpr (sizeof(R) == 4)
+            if (imag) return launch_sweep<P, kRoundTripImagEnergy>(c, axis, a);
+        return launch_sweep<P, kRoundTripEnergy>(c, axis, a);
     }
     if constexpr (sizeof(R) == 4)
         if (imag) return launch_sweep<P, kRoundTripImag>(c, axis, a);
@@ -485,15 +549,21 @@ int norm2_impl(sfd_ctx* c, const void* psi, double* dev_out)
     hipLaunchKernelGGL(k_norm2<S>, dim3((unsigned)nb), dim3(256), 0, c->stream,
                        static_cast<const cx<S>*>(psi), c->nsite, c->partial);
     SFD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nb, 1, dev_out, 0.0,
-                       (double*)nullptr);
+    launch_finalize(c->stream, c->partial, nb, 1, dev_out);
     SFD_HIP(hipGetLastError());
     return SFD_OK;
 }
 
 template <typename P>
+// nsteps split steps.  With `ehist` (device, 8 doubles per step), the first
+// half-step sweeps of step s also sum the energy of the state they load and
+// write the sums of that state (the state before step s) to ehist[8 s ..]:
+// [0] sum xi_x^2 |psi_hat_x|^2, [1] sum |psi|^2, [2] sum V|psi|^2,
+// [3] sum |psi|^4, [3 + d] sum xi_d^2 |psi_hat_d|^2 for d = 1, 2.  The y and
+// z sums are taken after the x (and y) flow, which leaves them unchanged
+// only in real time with Omega = 0 (the caller's condition).
 int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt,
-               int64_t nsteps, int order, int mode)
+               int64_t nsteps, int order, int mode, double* ehist = nullptr)
 {
     using R = typename P::R;
     using S = typename P::S;
@@ -505,21 +575,46 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
     const int nlflags = kFlagNL | (imag ? kFlagImag : 0);
     // Reduction workspace for the per-step renormalisation.
     if (imag) SFD_TRY(ensure_partial(c, 5 * sweep_nblocks(c, D - 1)));
+    // Energy partials: one region per axis, all finalized after the step's
+    // first half in one launch.
+    double* epart[3] = {nullptr, nullptr, nullptr};
+    if (ehist) {
+        long long off[3], tot = 0;
+        for (int ax = 0; ax < D; ++ax) {
+            off[ax] = tot;
+            tot += 4 * sweep_nblocks(c, ax);
+        }
+        SFD_TRY(ensure_buf(c, &c->epartial, &c->epartial_cap, tot));
+        for (int ax = 0; ax < D; ++ax) epart[ax] = c->epartial + off[ax];
+    }
     bool pending = false;
     for (int64_t s = 0; s < nsteps; ++s) {
+        FinalizeSets fs{};
         for (int h = 0; h < order; ++h) {
             for (int ax = 0; ax < D; ++ax) {
                 int flags = 0;
                 if (ax == 0 && h == 0 && pending) flags |= kFlagScale;
                 if (ax == D - 1 && h == 0) flags |= nlflags;
                 if (ax == D - 1 && h == order - 1 && imag) flags |= kFlagNorm;
-                SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, flags, beta, dt));
+                const bool en = ehist && h == 0;
+                if (en && ax == 0) flags |= kFlagPointwise;
+                SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, flags, beta, dt, nullptr,
+                                      en ? epart[ax] : nullptr));
+                if (en) {
+                    fs.partial[ax] = epart[ax];
+                    fs.nb[ax] = c->last_nblocks;
+                    fs.K[ax] = ax == 0 ? 4 : 1;
+                    fs.out[ax] = ehist + 8 * s + (ax == 0 ? 0 : 3 + ax);
+                }
+            }
+            if (ehist && h == 0) {
+                hipLaunchKernelGGL(k_finalize_sets, dim3(fs.rows()), dim3(kFinalizeThreads), 0, c->stream, fs);
+                SFD_HIP(hipGetLastError());
             }
         }
         if (imag) {
             const long long nb = c->last_nblocks;  // partials of the norm sweep just launched
-            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nb, 1, c->scal + 1,
-                               c->dvol(), c->scal);
+            launch_finalize(c->stream, c->partial, nb, 1, c->scal + 1, c->dvol(), c->scal);
             SFD_HIP(hipGetLastError());
             pending = true;
         }
@@ -533,6 +628,29 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
     return SFD_OK;
 }
 
+// (EOm, ED, Eb, E, mu) from the Parseval and pointwise sums of a state:
+// kin[d] = sum xi_d^2 |psi_hat_d|^2, rot[d] = sum_lines coord sum xi_d |psi_hat_d|^2
+// (d = x, y), pw = (sum |psi|^2, sum V|psi|^2, sum |psi|^4).
+// Parseval: sum_sites |d_ax psi|^2 = (1/n_ax) sum xi^2 |psi_hat|^2, and
+// sum_sites Re(i conj(psi) y d_x psi) = -(1/n_x) sum_lines y sum xi |psi_hat|^2.
+void combine_energy(const sfd_ctx* c, const double* kin, const double* rot, const double* pw, double cD,
+                    double beta, double Om, double* out5)
+{
+    double k = 0.0;
+    for (int ax = 0; ax < c->ndim; ++ax) k += kin[ax] / (double)c->n[ax];
+    double r = 0.0;
+    if (c->ndim >= 2) r = -rot[0] / (double)c->n[0] + rot[1] / (double)c->n[1];
+    const double dv = c->dvol();
+    const double EOm = Om * r * dv;
+    const double ED = (-cD * k + pw[1]) * dv;
+    const double Eb = 0.5 * beta * pw[2] * dv;
+    out5[0] = EOm;
+    out5[1] = ED;
+    out5[2] = Eb;
+    out5[3] = -EOm + ED + Eb;
+    out5[4] = -EOm + ED + Eb * 2.0;
+}
+
 template <typename P>
 int energy_impl(sfd_ctx* c, const void* psi, const void* V, double cD, double beta, double Om, double* out5)
 {
@@ -541,6 +659,13 @@ int energy_impl(sfd_ctx* c, const void* psi, const void* V, double cD, double be
     (void)sizeof(R);
     (void)sizeof(S);
     double sums[3][5];
+    long long off[3], tot = 0;
+    for (int ax = 0; ax < c->ndim; ++ax) {
+        off[ax] = tot;
+        tot += 5 * sweep_nblocks(c, ax);
+    }
+    SFD_TRY(ensure_partial(c, tot));
+    FinalizeSets fs{};
     for (int ax = 0; ax < c->ndim; ++ax) {
         SweepArgs a;
         base_args<P>(c, ax, a);
@@ -550,31 +675,59 @@ int energy_impl(sfd_ctx* c, const void* psi, const void* V, double cD, double be
         a.flags = ax == 0 ? kFlagPointwise : 0;
         if (ax == 0 && c->ndim >= 2) { a.coord = c->coord[1]; a.coord_mod = c->n[1]; }
         if (ax == 1) a.coord = c->coord[0];
-        const long long nb = sweep_nblocks(c, ax);
-        SFD_TRY(ensure_partial(c, 5 * nb));
-        a.partial = c->partial;
+        a.partial = c->partial + off[ax];
         SFD_TRY(launch_sweep<P, kEnergy>(c, ax, a));
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nb, 5, c->scal + 8, 0.0,
-                           (double*)nullptr);
-        SFD_HIP(hipGetLastError());
-        SFD_HIP(hipMemcpyAsync(sums[ax], c->scal + 8, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        SFD_HIP(hipStreamSynchronize(c->stream));
+        fs.partial[ax] = a.partial;
+        fs.nb[ax] = c->last_nblocks;
+        fs.K[ax] = 5;
+        fs.out[ax] = c->scal + 8 + 5 * ax;
     }
-    // Parseval: sum_sites |d_ax psi|^2 = (1/n_ax) sum xi^2 |psi_hat|^2, and
-    // sum_sites Re(i conj(psi) y d_x psi) = -(1/n_x) sum_lines y sum xi |psi_hat|^2.
-    double kin = 0.0;
-    for (int ax = 0; ax < c->ndim; ++ax) kin += sums[ax][0] / (double)c->n[ax];
-    double rot = 0.0;
-    if (c->ndim >= 2) rot = -sums[0][1] / (double)c->n[0] + sums[1][1] / (double)c->n[1];
-    const double dv = c->dvol();
-    const double EOm = Om * rot * dv;
-    const double ED = (-cD * kin + sums[0][3]) * dv;
-    const double Eb = 0.5 * beta * sums[0][4] * dv;
-    out5[0] = EOm;
-    out5[1] = ED;
-    out5[2] = Eb;
-    out5[3] = -EOm + ED + Eb;
-    out5[4] = -EOm + ED + Eb * 2.0;
+    hipLaunchKernelGGL(k_finalize_sets, dim3(fs.rows()), dim3(kFinalizeThreads), 0, c->stream, fs);
+    SFD_HIP(hipGetLastError());
+    SFD_HIP(hipMemcpyAsync(sums, c->scal + 8, 5 * c->ndim * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SFD_HIP(hipStreamSynchronize(c->stream));
+    const double kin[3] = {sums[0][0], c->ndim > 1 ? sums[1][0] : 0.0, c->ndim > 2 ? sums[2][0] : 0.0};
+    const double rot[2] = {sums[0][1], c->ndim > 1 ? sums[1][1] : 0.0};
+    const double pw[3] = {sums[0][2], sums[0][3], sums[0][4]};
+    combine_energy(c, kin, rot, pw, cD, beta, Om, out5);
+    return SFD_OK;
+}
+
+// solve!'s loop: nsteps of (energy of the state, split step).  In real time
+// with Omega = 0 the energy comes out of each step's first-half sweeps
+// (split_impl with a history): the y and z sums are taken after the x (and
+// y) flow, which is unitary along its axis and commutes with the y/z
+// transforms, so the sums are the state's own.  With rotation the x flow's
+// phase depends on y, and in imaginary time the flows damp; each step is
+// then preceded by a separate energy evaluation.  energies: host, 5 per step.
+template <typename P>
+int split_energy_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt,
+                      int64_t nsteps, int order, int mode, double* energies)
+{
+    if (Om != 0.0 || mode != SFD_REAL_TIME) {
+        for (int64_t s = 0; s < nsteps; ++s) {
+            SFD_TRY(energy_impl<P>(c, psi, V, cD, beta, Om, energies + 5 * s));
+            SFD_TRY(split_impl<P>(c, psi, V, cD, beta, Om, dt, 1, order, mode));
+        }
+        return SFD_OK;
+    }
+    constexpr int64_t kChunk = 4096;  // steps per history buffer (256 KiB)
+    std::vector<double> h;
+    for (int64_t s0 = 0; s0 < nsteps; s0 += kChunk) {
+        const int64_t m = std::min(kChunk, nsteps - s0);
+        SFD_TRY(ensure_buf(c, &c->ehist, &c->ehist_cap, 8 * m));
+        SFD_TRY(split_impl<P>(c, psi, V, cD, beta, Om, dt, m, order, mode, c->ehist));
+        h.resize(8 * (size_t)m);
+        SFD_HIP(hipMemcpyAsync(h.data(), c->ehist, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        SFD_HIP(hipStreamSynchronize(c->stream));
+        for (int64_t s = 0; s < m; ++s) {
+            const double* r = h.data() + 8 * s;
+            const double kin[3] = {r[0], r[4], r[5]};
+            const double rot[2] = {0.0, 0.0};
+            const double pw[3] = {r[1], r[2], r[3]};
+            combine_energy(c, kin, rot, pw, cD, beta, Om, energies + 5 * (s0 + s));
+        }
+    }
     return SFD_OK;
 }
 
@@ -663,7 +816,7 @@ int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target
     hipLaunchKernelGGL(k_inner<S>, dim3((unsigned)nb), dim3(256), 0, c->stream, static_cast<const cx<S>*>(target),
                        static_cast<const cx<S>*>(psi.p), ns, partd);
     SFD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, partd, nb, 2, accd, 0.0, (double*)nullptr);
+    launch_finalize(c->stream, partd, nb, 2, accd);
     SFD_HIP(hipGetLastError());
     double cs[2];
     SFD_HIP(hipMemcpyAsync(cs, accd, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -798,6 +951,8 @@ void release(sfd_ctx* c)
     }
     for (auto& p : c->phases) (void)hipFree(p.dev);
     if (c->partial) (void)hipFree(c->partial);
+    if (c->epartial) (void)hipFree(c->epartial);
+    if (c->ehist) (void)hipFree(c->ehist);
     if (c->scal) (void)hipFree(c->scal);
 }
 
@@ -1357,6 +1512,21 @@ int sfd_split_step(sfd_ctx* ctx, void* psi, const void* V, double coeffD, double
     return SFD_BY_PREC(ctx, split_impl, ctx, psi, V, coeffD, beta, Omega, dt, nsteps, order, mode);
 }
 
+int sfd_split_step_energy(sfd_ctx* ctx, void* psi, const void* V, double coeffD, double beta, double Omega,
+                          double dt, int64_t nsteps, int order, int mode, double* energies)
+{
+    SFD_TRY(check_ctx(ctx));
+    if (!psi) return fail(SFD_ERR_NULL, "sfd_split_step_energy: psi is null");
+    if (!energies && nsteps > 0) return fail(SFD_ERR_NULL, "sfd_split_step_energy: energies is null");
+    if (order != 1 && order != 2) return fail(SFD_ERR_ARG, "order must be 1 or 2 (got %d)", order);
+    if (mode != SFD_REAL_TIME && mode != SFD_IMAG_TIME) return fail(SFD_ERR_ARG, "mode must be 0 or 1");
+    if (nsteps < 0) return fail(SFD_ERR_ARG, "nsteps must be >= 0");
+    if (!std::isfinite(coeffD) || !std::isfinite(beta) || !std::isfinite(Omega) || !std::isfinite(dt))
+        return fail(SFD_ERR_ARG, "non-finite scalar argument");
+    return SFD_BY_PREC(ctx, split_energy_impl, ctx, psi, V, coeffD, beta, Omega, dt, nsteps, order, mode,
+                       energies);
+}
+
 int sfd_lap_rot_axis(sfd_ctx* ctx, void* psi, int axis, double coeffD, double Omega, double tau, int mode)
 {
     SFD_TRY(check_ctx(ctx));
@@ -1559,8 +1729,7 @@ int sfd_inner(sfd_ctx* ctx, const void* a, const void* b, double* out2)
                            static_cast<const cx<float>*>(a), static_cast<const cx<float>*>(b), ctx->nsite,
                            ctx->partial);
     SFD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, ctx->stream, ctx->partial, nb, 2, ctx->scal + 4, 0.0,
-                       (double*)nullptr);
+    launch_finalize(ctx->stream, ctx->partial, nb, 2, ctx->scal + 4);
     SFD_HIP(hipGetLastError());
     SFD_HIP(hipMemcpyAsync(out2, ctx->scal + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     SFD_HIP(hipStreamSynchronize(ctx->stream));
@@ -1583,9 +1752,8 @@ int sfd_project(sfd_ctx* ctx, const double* field, const double* basis, int nbas
                        ctx->nsite, static_cast<double*>(part.p));
     SFD_HIP(hipGetLastError());
     for (int j = 0; j < nbasis; ++j) {
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, ctx->stream,
-                           static_cast<const double*>(part.p) + (size_t)j * nb, nb, 1,
-                           static_cast<double*>(res.p) + j, 0.0, (double*)nullptr);
+        launch_finalize(ctx->stream, static_cast<const double*>(part.p) + (size_t)j * nb, nb, 1,
+                        static_cast<double*>(res.p) + j);
         SFD_HIP(hipGetLastError());
     }
     SFD_HIP(hipMemcpyAsync(out, res.p, (size_t)nbasis * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

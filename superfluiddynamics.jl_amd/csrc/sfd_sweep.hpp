@@ -30,14 +30,24 @@ enum SweepMode : int {
     kInverse = 2,    // inverse only (normalised by 1/N), spectrum read in natural order
     kEnergy = 3,     // forward only, spectral and pointwise sums, nothing stored
     kRoundTripImag = 4,  // kRoundTrip for imaginary time (FP32 sweeps: real decay tables)
+    kRoundTripEnergy = 5,      // kRoundTrip that also sums the energy of the loaded state
+    kRoundTripImagEnergy = 6,  // kRoundTripImag that also sums the energy of the loaded state
 };
+
+constexpr bool is_round_trip(int mode)
+{
+    return mode == kRoundTrip || mode == kRoundTripImag || mode == kRoundTripEnergy ||
+           mode == kRoundTripImagEnergy;
+}
+constexpr bool rt_imag(int mode) { return mode == kRoundTripImag || mode == kRoundTripImagEnergy; }
+constexpr bool rt_energy(int mode) { return mode == kRoundTripEnergy || mode == kRoundTripImagEnergy; }
 
 enum SweepFlags : int {
     kFlagNL = 1,        // epilogue: psi *= exp(-i (V + beta|psi|^2) dt)
     kFlagImag = 2,      // imaginary time: NL factor exp(-(V + beta|psi|^2) dt)
     kFlagNorm = 4,      // epilogue: per-workgroup sum |psi|^2 into partial[]
     kFlagScale = 8,     // prologue: psi *= *scale (pending renormalisation)
-    kFlagPointwise = 16 // energy mode: also sum |psi|^2, V|psi|^2, |psi|^4
+    kFlagPointwise = 16 // energy sums: also sum |psi|^2, V|psi|^2, |psi|^4
 };
 
 struct SweepArgs {
@@ -64,6 +74,7 @@ struct SweepArgs {
     long long ntiles;       // workgroup tiles of the launch
     int ntx;                // strided flavour: x-tiles per "other" index
     void* save;             // kFlagNL: the state before the nonlinear phase is also stored here
+    double* epartial;       // energy round trips: per-workgroup sums [k * nblocks + block], k < 4
 };
 
 template <typename R, int N1, int N2, int TS, int G>
@@ -338,11 +349,15 @@ __device__ __forceinline__ void inverse_ba(cx<R> (&w)[N1 / N2][N2], cx<R> (&v)[N
 // inverse, then the fused epilogue (nonlinear phase, norm partial sum).
 // IMAG selects the imaginary-time form of the FP32 phase multiply and NL
 // factor at compile time (FP64 arithmetic reads kFlagImag at run time).
-template <typename S, typename R, int N1, int N2, int TS, int G, bool IMAG>
+// EN adds the energy sums of the state entering the sweep (after a pending
+// scale) to eacc[4], in FP64: [0] sum xi^2 |psi_hat|^2 over the transformed
+// line, and with kFlagPointwise in a contiguous (x) sweep [1] sum |psi|^2,
+// [2] sum V|psi|^2, [3] sum |psi|^4 (vp must then hold V).
+template <typename S, typename R, int N1, int N2, int TS, int G, bool IMAG, bool EN = false>
 __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1], const TileGeom& gm,
                                                 const SweepArgs& a, cx<R>* lds, const cx<R>* twA,
                                                 const cx<R>* twB, const cx<R>* phase, int c, int g, int t,
-                                                double& norm_acc)
+                                                double& norm_acc, double* eacc = nullptr)
 {
     constexpr int RB = N1 / N2;
     const int flags = a.flags;
@@ -351,8 +366,35 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
 #pragma unroll
         for (int j = 0; j < N1; ++j) v[j] = cscale(v[j], sc);
     }
+    if constexpr (EN && TS == 1) {  // the x sweep (contiguous flavour) takes the pointwise sums
+        if (flags & kFlagPointwise) {
+#pragma unroll
+            for (int j = 0; j < N1; ++j) {
+                const double re = v[j].re, im = v[j].im;
+                const double a2 = re * re + im * im;
+                eacc[1] += a2;
+                eacc[2] += (double)vp[j] * a2;
+                eacc[3] += a2 * a2;
+            }
+        }
+    }
     cx<R> w[RB][N2];
     forward_ab<R, N1, N2, TS, G>(v, w, lds, twA, c, g, t);
+    if constexpr (EN) {
+        // xi_k = kk * xi_1 with kk the signed frequency of k (the wavenumber
+        // table's own product)
+        constexpr int N = N1 * N2;
+        const double dk = a.xi[1];
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const int k = t + N2 * r + N1 * k2;
+                const double x = (double)(k < N / 2 ? k : k - N) * dk;
+                const double re = w[r][k2].re, im = w[r][k2].im;
+                eacc[0] += x * x * (re * re + im * im);
+            }
+    }
     const cx<R>* __restrict__ ph = phase + t;
     if constexpr (sizeof(R) == 4) {
         // FP32: X m/N with X/N exact (N a power of two).  Real-time tables
@@ -426,15 +468,43 @@ __global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
     cx<R> v[N1];
     R vp[N1];
 
-    if constexpr (MODE == kRoundTrip || MODE == kRoundTripImag) {
+    if constexpr (is_round_trip(MODE)) {
+        constexpr bool EN = rt_energy(MODE);
         load_line<S, R, N1>(v, psi, gm);
-        if (flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
+        if ((flags & kFlagNL) || (EN && TS == 1 && (flags & kFlagPointwise)))
+            load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
         double nacc = 0.0;
-        round_trip_tile<S, R, N1, N2, TS, G, MODE == kRoundTripImag>(v, vp, gm, a, lds, twA, twB,
-                                                                     row_phase<R>(a, gm), c, g, t, nacc);
+        double eacc[4] = {0, 0, 0, 0};
+        if constexpr (EN) {
+            // Energy sweeps read the twiddle and phase tables (one phase row:
+            // Omega = 0) from LDS: with the energy sums' registers live the
+            // compiler no longer batches the tables' global loads and would
+            // wait on each in turn.
+            __shared__ cx<R> tabs[3 * N];
+            const cx<R>* gtw = static_cast<const cx<R>*>(a.tw);
+            const cx<R>* gph = static_cast<const cx<R>*>(a.phase);
+#pragma unroll
+            for (int k = 0; k < (3 * N + Sh::NT - 1) / Sh::NT; ++k) {
+                const int i = tid + k * Sh::NT;
+                if ((3 * N) % Sh::NT == 0 || i < 3 * N) tabs[i] = i < 2 * N ? gtw[i] : gph[i - 2 * N];
+            }
+            lds_barrier();
+            round_trip_tile<S, R, N1, N2, TS, G, rt_imag(MODE), true>(v, vp, gm, a, lds, tabs + t, tabs + N + t,
+                                                                     tabs + 2 * N, c, g, t, nacc, eacc);
+        } else {
+            round_trip_tile<S, R, N1, N2, TS, G, rt_imag(MODE), false>(v, vp, gm, a, lds, twA, twB,
+                                                                      row_phase<R>(a, gm), c, g, t, nacc);
+        }
         if (flags & kFlagNorm) {
             double acc1[1] = {nacc};
             block_partials<R, Sh::NT, 1>(acc1, a.partial, nblocks, tile);
+        }
+        if constexpr (EN) {
+            if (!gm.active) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) eacc[k] = 0.0;
+            }
+            block_partials<R, Sh::NT, 4>(eacc, a.epartial, nblocks, tile);
         }
         store_line<S, R, N1>(v, psi, gm);
     } else if constexpr (MODE == kForward) {

@@ -123,10 +123,12 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
     static bool launch(int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a, bool persistent)
     {
         if (idx == 0) {
+            // energy round trips run k_sweep: with the sums' registers the
+            // pipelined kernel drops to two wavefronts per SIMD and is slower
             if constexpr (V0::P && (MODE == kRoundTrip || MODE == kRoundTripImag)) {
                 if (persistent) {
-                    hipLaunchKernelGGL((k_sweep_pipe<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE == kRoundTripImag>),
-                                       grid, block, 0, s, a);
+                    hipLaunchKernelGGL((k_sweep_pipe<S, R, V0::N1, V0::N2, V0::TS, V0::G, rt_imag(MODE)>), grid,
+                                       block, 0, s, a);
                     return true;
                 }
             }

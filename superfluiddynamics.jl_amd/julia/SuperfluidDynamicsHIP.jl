@@ -211,4 +211,44 @@ function SuperfluidDynamics.energy(n::Union{NumModelSplit1{<:HipField},NumModelS
     return EΩ, EΔ, Eβ, E
 end
 
+# solve! of NumModels.jl:23-46 for the split models on HIP fields.  The
+# iterations up to each backup (it % freqbckp == 0) run in one
+# sfd_split_step_energy call: with Ω = 0 every step's sweeps also give the
+# energy of the state before it, so no separate energy pass is made (with
+# Ω ≠ 0 the library evaluates the energy before each step).
+function SuperfluidDynamics.solve!(n::Union{NumModelSplit1{<:HipField},NumModelSplit2{<:HipField}};
+                                   istart=1, plot=false)
+    plot && error("plot=true: use the reference's solve! on host arrays")
+    order = n isa NumModelSplit2 ? 2 : 1
+    c = context(n.f)
+    p = n.param
+    res = Tuple{Int64,Int64,Float64,Float64,Float64,Float64}[]
+    total = 0
+    SuperfluidDynamics.write!(n.writers; prefix="res", istep=istart, Δt=n.Δt)
+    it, last = istart, istart + n.niter
+    while it <= last
+        stop = min(last, it + mod(-it, n.freqbckp))
+        k = stop - it + 1
+        E = zeros(Cdouble, 5, k)
+        check(ccall((:sfd_split_step_energy, libsfd), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Int64, Cint, Cint,
+                     Ptr{Cdouble}),
+                    c.handle, devptr(n.f.ϕ), vptr(p), p.coeffΔ, p.β, p.Ω, n.Δt, k, order, SFD_REAL_TIME, E))
+        for j in 1:k
+            EΩ, EΔ, Eβ, Et, μ = E[:, j]
+            println("iteration $(it + j - 1)")
+            ndims(n.f.ϕ) == 2 && println("Chemical potential : $(μ)")
+            println("Angular Momentum Energy : $(EΩ)")
+            println("Kinetic + Potential Energy : $(EΔ)")
+            println("Interaction Energy : $(Eβ)")
+            println("Total Energy: $(Et)")
+            total += order
+            push!(res, (order, total, EΩ, EΔ, Eβ, Et))
+        end
+        stop % n.freqbckp == 0 && SuperfluidDynamics.write!(n.writers; prefix="res", istep=stop, Δt=n.Δt)
+        it = stop + 1
+    end
+    return res
+end
+
 end # module

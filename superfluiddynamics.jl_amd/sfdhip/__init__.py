@@ -4,7 +4,7 @@ from ._lib import SfdError, ShapeError, ArgError, build, load, EXPORTS, LIB_PATH
 from .api import (  # noqa: F401
     HipArray, ComplexField, RealField, Grid, Field, norm, normalize, normalize_,
     PotentialZero, PotentialQuadratic, PotentialArray, GrossPitaevskiiParameters,
-    NumModelSplit1, NumModelSplit2, NumModelSplitImag, timeStep, energy, energy_all, solve,
+    NumModelSplit1, NumModelSplit2, NumModelSplitImag, timeStep, energy, energy_all, solve, steps_with_energy,
     solveLapRot, solveNL, BatchImagSolver, adjoint_gradient, FidelityLoss, fidelity_loss, project, inner,
     mul_x, mul_y, mul_z, ldiv_x, ldiv_y, ldiv_z, grid_x, grid_y, grid_z, linrange_coords, fftfreq,
 )

@@ -31,7 +31,7 @@ SFD_BACKWARD = 1
 #: every symbol include/sfd.h declares
 EXPORTS = (
     "sfd_ctx_create", "sfd_ctx_destroy", "sfd_ctx_set_stream", "sfd_split_step",
-    "sfd_lap_rot", "sfd_lap_rot_axis", "sfd_solve_nl", "sfd_norm2", "sfd_normalize", "sfd_energy", "sfd_fft_axis", "sfd_alloc", "sfd_free",
+    "sfd_lap_rot", "sfd_lap_rot_axis", "sfd_solve_nl", "sfd_norm2", "sfd_normalize", "sfd_energy", "sfd_split_step_energy", "sfd_fft_axis", "sfd_alloc", "sfd_free",
     "sfd_upload", "sfd_download", "sfd_synchronize", "sfd_last_error", "sfd_version",
     "sfd_autotune", "sfd_ctx_get_variant", "sfd_ctx_set_variant",
     "sfd_batch_create", "sfd_batch_destroy", "sfd_batch_imag_run", "sfd_batch_autotune", "sfd_adjoint",
@@ -87,6 +87,7 @@ def load():
         "sfd_norm2": ([vp, vp, dp], c_int),
         "sfd_normalize": ([vp, vp, dp], c_int),
         "sfd_energy": ([vp, vp, vp, c_dbl, c_dbl, c_dbl, dp], c_int),
+        "sfd_split_step_energy": ([vp, vp, vp, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_int, c_int, dp], c_int),
         "sfd_fft_axis": ([vp, vp, vp, c_int, c_int], c_int),
         "sfd_alloc": ([vp, c_sz, ctypes.POINTER(vp)], c_int),
         "sfd_free": ([vp, vp], c_int),

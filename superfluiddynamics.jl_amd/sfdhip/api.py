@@ -420,19 +420,40 @@ def energy_all(n: _SplitBase):
     return tuple(out)
 
 
+def steps_with_energy(n: _SplitBase, k: int):
+    """k iterations of (energy_all(n), timeStep(n)) in one call
+    (sfd_split_step_energy): a (k, 5) array of (EΩ, EΔ, Eβ, E, μ) of the
+    state before each step.  In real time with Ω = 0 the energies come out of
+    the steps' own sweeps."""
+    c = n.f.ctx()
+    p = n.param
+    out = np.zeros((int(k), 5))
+    L.check(c.lib.sfd_split_step_energy(c.h, ctypes.c_void_p(n.f.ϕ.data_ptr()), _vptr(p), p.coeffΔ, p.β, p.Ω,
+                                        n.Δt, int(k), n.order, n.mode,
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
 def solve(n: _SplitBase, istart: int = 1, verbose: bool = False):
     """solve!(n; istart) (NumModels.jl:23-46) without the HDF5/VTK writers:
     niter+1 iterations of (energy, timeStep!); returns the history of
-    (nstep, total, EΩ, EΔ, Eβ, E)."""
+    (nstep, total, EΩ, EΔ, Eβ, E).  The iterations run in one
+    sfd_split_step_energy call; the printed lines follow it."""
+    hist = steps_with_energy(n, n.niter + 1)
     res = []
     total = 0
-    for it in range(istart, istart + n.niter + 1):
+    for i, it in enumerate(range(istart, istart + n.niter + 1)):
+        EΩ, EΔ, Eβ, E, μ = (float(x) for x in hist[i])
         if verbose:
             print(f"iteration {it}")
-        E = energy(n, verbose)
-        k = timeStep(n)
-        total += k
-        res.append((k, total) + tuple(E))
+            if n.f.g.ndims == 2:
+                print(f"Chemical potential : {μ}")
+            print(f"Angular Momentum Energy : {EΩ}")
+            print(f"Kinetic + Potential Energy : {EΔ}")
+            print(f"Interaction Energy : {Eβ}")
+            print(f"Total Energy: {E}")
+        total += n.order
+        res.append((n.order, total, EΩ, EΔ, Eβ, E))
     return res
 
 
