@@ -367,7 +367,9 @@ class NumModelSplitImag(_SplitBase):
     """Imaginary-time split step (normalised gradient flow): the Split1/Split2
     sweeps with every exponent real, then normalize! after each step.  Its fixed
     point is the ground state that NumModelBackwardEuler reaches
-    (backward-euler.jl:72-85), up to the O(Δt^order) splitting error."""
+    (backward-euler.jl:72-85) up to a splitting error: with Strang, O(Δt²)
+    in the energy and O(Δt) in the state and μ (the nonlinear factor uses
+    |ψ|² after the first half kinetic flow)."""
     mode = L.SFD_IMAG_TIME
 
     def __init__(self, f, param, Δt, niter, freqbckp, order: int = 2):

@@ -89,8 +89,7 @@ def test_imag_time_bao_du_2d(gpu):
     β = 200, box [-8,8)×[-4,4), 256², V = ½(x² + γy² y²) with γy = 4 in the
     γ² convention of the paper; E = 11.1563, μ = 16.3377.  The imaginary-time
     Strang split at Δt = 0.02 reaches E within 1e-4 and μ within 3e-3 (its
-    fixed point carries an O(Δt²) splitting error; the literature μ itself
-    differs from the converged discrete μ at the 2e-3 level)."""
+    fixed point carries a splitting error, O(Δt²) in E and O(Δt) in μ)."""
     g = gpu.Grid((256, 256), ((-8, 8), (-4, 4)))
     f = gpu.Field(g)
     x, y = f.coords()
