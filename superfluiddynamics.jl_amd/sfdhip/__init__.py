@@ -8,5 +8,6 @@ from .api import (  # noqa: F401
     solveLapRot, solveNL, BatchImagSolver, adjoint_gradient, FidelityLoss, fidelity_loss, project, inner,
     mul_x, mul_y, mul_z, ldiv_x, ldiv_y, ldiv_z, grid_x, grid_y, grid_z, linrange_coords, fftfreq,
 )
+from .io import WriterSave, write, read, CheckpointError  # noqa: F401
 
 __all__ = [n for n in dir() if not n.startswith("_")]

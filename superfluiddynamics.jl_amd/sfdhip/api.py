@@ -436,26 +436,42 @@ def steps_with_energy(n: _SplitBase, k: int):
     return out
 
 
-def solve(n: _SplitBase, istart: int = 1, verbose: bool = False):
-    """solve!(n; istart) (NumModels.jl:23-46) without the HDF5/VTK writers:
-    niter+1 iterations of (energy, timeStep!); returns the history of
-    (nstep, total, EΩ, EΔ, Eβ, E).  The iterations run in one
+def solve(n: _SplitBase, istart: int = 1, verbose: bool = False, writers=()):
+    """solve!(n; istart) (NumModels.jl:23-46): niter+1 iterations of
+    (energy, timeStep!); returns the history of (nstep, total, EΩ, EΔ, Eβ, E).
+    ``writers`` (``sfdhip.io.WriterSave`` objects; the reference's VTK writer
+    is not built) save "res-{istep}.h5" at istart and after every iteration
+    it with it % freqbckp == 0.  The iterations between two saves run in one
     sfd_split_step_energy call; the printed lines follow it."""
-    hist = steps_with_energy(n, n.niter + 1)
+    from . import io as _io
+    writers = list(writers)
+
+    def save(istep):
+        for w in writers:
+            _io.write(w, prefix="res", istep=istep, Δt=n.Δt)
+
+    save(istart)
     res = []
     total = 0
-    for i, it in enumerate(range(istart, istart + n.niter + 1)):
-        EΩ, EΔ, Eβ, E, μ = (float(x) for x in hist[i])
-        if verbose:
-            print(f"iteration {it}")
-            if n.f.g.ndims == 2:
-                print(f"Chemical potential : {μ}")
-            print(f"Angular Momentum Energy : {EΩ}")
-            print(f"Kinetic + Potential Energy : {EΔ}")
-            print(f"Interaction Energy : {Eβ}")
-            print(f"Total Energy: {E}")
-        total += n.order
-        res.append((n.order, total, EΩ, EΔ, Eβ, E))
+    it, last = istart, istart + n.niter
+    while it <= last:
+        stop = min(last, it + (-it) % n.freqbckp) if writers else last
+        hist = steps_with_energy(n, stop - it + 1)
+        for j in range(stop - it + 1):
+            EΩ, EΔ, Eβ, E, μ = (float(x) for x in hist[j])
+            if verbose:
+                print(f"iteration {it + j}")
+                if n.f.g.ndims == 2:
+                    print(f"Chemical potential : {μ}")
+                print(f"Angular Momentum Energy : {EΩ}")
+                print(f"Kinetic + Potential Energy : {EΔ}")
+                print(f"Interaction Energy : {Eβ}")
+                print(f"Total Energy: {E}")
+            total += n.order
+            res.append((n.order, total, EΩ, EΔ, Eβ, E))
+        if stop % n.freqbckp == 0:
+            save(stop)
+        it = stop + 1
     return res
 
 

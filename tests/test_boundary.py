@@ -1,5 +1,6 @@
 """The C-ABI boundary without a GPU: the library loads, exports every symbol
-include/sfd.h declares, and rejects bad arguments before touching HIP."""
+include/sfd.h declares, and rejects bad arguments before touching HIP (and
+the checkpoint library exports what include/sfd_io.h declares)."""
 import ctypes
 import os
 import re
@@ -10,8 +11,8 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    txt = open(os.path.join(ROOT, "include", "sfd.h")).read()
+def _declared(header="sfd.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(sfd_[a-z0-9_]+)\s*\(", txt)))
 
@@ -28,6 +29,21 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     assert lib.sfd_version() == 100
     assert lib.sfd_last_error() is not None
+
+
+def test_checkpoint_library_exports_every_declared_symbol():
+    """include/sfd_io.h is served by libsfdio.so (built when the HDF5 C
+    library headers are present)."""
+    from sfdhip import io
+    if not os.path.exists(io.IO_PATH):
+        pytest.skip("libsfdio.so not built (no HDF5 headers at build time)")
+    lib = io.load_io()
+    names = _declared("sfd_io.h")
+    assert names == ["sfd_io_last_error", "sfd_io_read", "sfd_io_write"]
+    for name in names:
+        assert hasattr(lib, name), name
+    assert lib.sfd_io_write(None, 2, None, 8, None) == -1
+    assert b"null" in lib.sfd_io_last_error()
 
 
 def _create(lib, ndim, n, lo, hi, prec):
