@@ -27,12 +27,17 @@ for p in (ROOT, os.path.join(ROOT, "superfluiddynamics.jl_amd")):
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 
-def bytes_per_site_update(prec: str, ndim: int = 3, with_v: bool = False) -> int:
-    """Algorithmic bytes per site-update: 2D one-axis round trips, each one
-    read and one write of the field; the nonlinear phase is fused."""
+def bytes_per_site_update(prec: str, ndim: int = 3, with_v: bool = False, passes=None) -> int:
+    """Bytes per site-update of a Strang step made of `passes` one-axis
+    sweeps, each one read and one write of the field, the nonlinear phase
+    fused (V read once when present).  2D passes is the reference's
+    structure; the step here makes 2D - 1 (the innermost axis's two round
+    trips share one sweep around the nonlinear phase)."""
     sc = 8 if prec == "f32" else 16
     sr = 4 if prec == "f32" else 8
-    return 2 * ndim * 2 * sc + (sr if with_v else 0)
+    if passes is None:
+        passes = 2 * ndim
+    return passes * 2 * sc + (sr if with_v else 0)
 
 
 def load_traffic(name: str):
@@ -258,10 +263,13 @@ def main(argv=None):
     for prec in ("f32", "f32x", "f64"):
         r = res[prec]
         sups = sites / (r["ms"] * 1e-3)
-        b = bytes_per_site_update("f64" if prec == "f64" else "f32")
+        p_ = "f64" if prec == "f64" else "f32"
+        b = bytes_per_site_update(p_, passes=5)
+        b6 = bytes_per_site_update(p_)
         out[prec] = dict(site_updates_per_s=sups, ms_per_step=r["ms"], bytes_per_site_update=b,
-                         step_hbm_frac=sups * b / HBM_PEAK,
-                         sweep_ms=dict(x=r["per_axis"][0], y=r["per_axis"][1], z=r["per_axis"][2]),
+                         step_hbm_frac=sups * b / HBM_PEAK, step_hbm_frac_6_pass_bytes=sups * b6 / HBM_PEAK,
+                         sweep_ms=dict(x=r["per_axis"][0], y=r["per_axis"][1], z=r["per_axis"][2],
+                                       z_fold_from_step=r["ms"] - 2 * (r["per_axis"][0] + r["per_axis"][1])),
                          variants=r["variants"])
 
     # Dominant kernel: the strided-axis sweep (y and z are 4 of the 6 launches).
@@ -319,6 +327,9 @@ def main(argv=None):
         "f32_over_f64_speedup": out["f64"]["ms_per_step"] / out["f32"]["ms_per_step"],
         "f32x_over_f64_speedup": out["f64"]["ms_per_step"] / out["f32x"]["ms_per_step"],
         "f32_step_hbm_frac": out["f32"]["step_hbm_frac"],
+        "f32_step_hbm_frac_note": "the step moves 5 passes x 16 B = 80 B per site-update (x, y, [z N z], y, x); "
+                                  "priced at the 6-pass 96 B of the reference's structure it would read "
+                                  f"{out['f32']['step_hbm_frac_6_pass_bytes']:.3f}",
         "f32_sweep_ms": out["f32"]["sweep_ms"],
         "f32_variants": out["f32"]["variants"],
         "f32_vs_f64_rel_l2": err32,

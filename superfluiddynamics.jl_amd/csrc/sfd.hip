@@ -485,8 +485,11 @@ template <typename P>
 // doubles), the sweep also writes the energy sums of the state it loads
 // there, 4 per workgroup (kRoundTripEnergy; kFlagPointwise in `flags` adds
 // the pointwise sums).
+// With `twice`, the sweep is K(tau) N K(tau) on this axis (kRoundTripTwice):
+// the nonlinear phase of `flags` between two round trips.
 int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, double cD, double Om, int imag,
-               int flags, double beta, double dt, void* save = nullptr, double* epartial = nullptr)
+               int flags, double beta, double dt, void* save = nullptr, double* epartial = nullptr,
+               bool twice = false)
 {
     using R = typename P::R;
     using S = typename P::S;
@@ -511,6 +514,17 @@ int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, doubl
     if (flags & kFlagNorm) {
         SFD_TRY(ensure_partial(c, 5 * sweep_nblocks(c, axis)));
         a.partial = c->partial;
+    }
+    if (twice) {
+        if (epartial) {
+            if (imag) return fail(SFD_ERR_ARG, "no energy sums in an imaginary-time double round trip");
+            a.epartial = epartial;
+            a.xi = c->xi[axis];
+            return launch_sweep<P, kRoundTripTwiceEnergy>(c, axis, a);
+        }
+        if constexpr (sizeof(R) == 4)
+            if (imag) return launch_sweep<P, kRoundTripTwiceImag>(c, axis, a);
+        return launch_sweep<P, kRoundTripTwice>(c, axis, a);
     }
     if (epartial) {
         a.epartial = epartial;
@@ -562,8 +576,16 @@ template <typename P>
 // [3] sum |psi|^4, [3 + d] sum xi_d^2 |psi_hat_d|^2 for d = 1, 2.  The y and
 // z sums are taken after the x (and y) flow, which leaves them unchanged
 // only in real time with Omega = 0 (the caller's condition).
+//
+// Strang steps with Omega = 0 run the axes as x, y, [z N z],
+// y, x (fold): all half-step flows are Fourier multipliers along different
+// axes and commute, so K(tau) = K_z K_y K_x may apply its axes in either
+// order, and the two flows of the innermost axis meet around the nonlinear
+// phase in one double round-trip sweep (2D - 1 sweeps per step instead of
+// 2D).  `fold = false` keeps the x, y, z, N, x, y, z order of
+// splitting.jl:149-154.
 int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt,
-               int64_t nsteps, int order, int mode, double* ehist = nullptr)
+               int64_t nsteps, int order, int mode, double* ehist = nullptr, bool fold = true)
 {
     using R = typename P::R;
     using S = typename P::S;
@@ -573,8 +595,9 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
     const int D = c->ndim;
     const double tau = order == 2 ? dt * 0.5 : dt;
     const int nlflags = kFlagNL | (imag ? kFlagImag : 0);
-    // Reduction workspace for the per-step renormalisation.
-    if (imag) SFD_TRY(ensure_partial(c, 5 * sweep_nblocks(c, D - 1)));
+    // Reduction workspace for the per-step renormalisation (its last sweep
+    // is on x when folded, else on the last axis).
+    if (imag) SFD_TRY(ensure_partial(c, 5 * std::max(sweep_nblocks(c, 0), sweep_nblocks(c, D - 1))));
     // Energy partials: one region per axis, all finalized after the step's
     // first half in one launch.
     double* epart[3] = {nullptr, nullptr, nullptr};
@@ -588,7 +611,36 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
         for (int ax = 0; ax < D; ++ax) epart[ax] = c->epartial + off[ax];
     }
     bool pending = false;
-    for (int64_t s = 0; s < nsteps; ++s) {
+    fold = fold && order == 2 && Om == 0.0 && D >= 2;
+    for (int64_t s = 0; s < nsteps && fold; ++s) {
+        FinalizeSets fs{};
+        auto record = [&](int ax) {
+            fs.partial[ax] = epart[ax];
+            fs.nb[ax] = c->last_nblocks;
+            fs.K[ax] = ax == 0 ? 4 : 1;
+            fs.out[ax] = ehist + 8 * s + (ax == 0 ? 0 : 3 + ax);
+        };
+        for (int ax = 0; ax < D - 1; ++ax) {
+            int flags = (ax == 0 && pending) ? kFlagScale : 0;
+            if (ehist && ax == 0) flags |= kFlagPointwise;
+            SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, flags, beta, dt, nullptr, epart[ax]));
+            if (ehist) record(ax);
+        }
+        SFD_TRY(round_trip<P>(c, psi, V, D - 1, tau, cD, Om, imag, nlflags, beta, dt, nullptr, epart[D - 1], true));
+        if (ehist) {
+            record(D - 1);
+            hipLaunchKernelGGL(k_finalize_sets, dim3(fs.rows()), dim3(kFinalizeThreads), 0, c->stream, fs);
+            SFD_HIP(hipGetLastError());
+        }
+        for (int ax = D - 2; ax >= 0; --ax)
+            SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, (ax == 0 && imag) ? kFlagNorm : 0, beta, dt));
+        if (imag) {
+            launch_finalize(c->stream, c->partial, c->last_nblocks, 1, c->scal + 1, c->dvol(), c->scal);
+            SFD_HIP(hipGetLastError());
+            pending = true;
+        }
+    }
+    for (int64_t s = 0; s < nsteps && !fold; ++s) {
         FinalizeSets fs{};
         for (int h = 0; h < order; ++h) {
             for (int ax = 0; ax < D; ++ax) {
@@ -808,7 +860,7 @@ int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target
     for (int64_t k = 0; k < nck; ++k) {
         SFD_HIP(hipMemcpyAsync(fld(ck.p, k), psi.p, fb, hipMemcpyDeviceToDevice, c->stream));
         const int64_t m = std::min<int64_t>(every, N - k * every);
-        SFD_TRY(split_impl<P>(c, psi.p, V, cD, beta, Om, dt, m, 2, SFD_REAL_TIME));
+        SFD_TRY(split_impl<P>(c, psi.p, V, cD, beta, Om, dt, m, 2, SFD_REAL_TIME, nullptr, false));
     }
     if (psi_final) SFD_HIP(hipMemcpyAsync(psi_final, psi.p, fb, hipMemcpyDeviceToDevice, c->stream));
 

@@ -132,8 +132,13 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
                     return true;
                 }
             }
-            hipLaunchKernelGGL((k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W>), grid, block, 0, s, a);
-            return true;
+            if constexpr (rt_twice(MODE) && V0::TS == 1) {
+                return false;  // the double round trip runs on a strided (y/z) axis only
+            } else {
+                hipLaunchKernelGGL((k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W>), grid, block, 0, s,
+                                   a);
+                return true;
+            }
         }
         return Dispatch<P, MODE, VL<Vs...>>::launch(idx - 1, grid, block, s, a, persistent);
     }
