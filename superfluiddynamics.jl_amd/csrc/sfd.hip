@@ -1055,6 +1055,9 @@ struct sfd_batch {
     hipGraphExec_t gexec = nullptr;
     int graph_k = 0;
     static constexpr int kDiffBlocks = 64;
+    // sweeps of one imaginary-time Strang step: x, [y N y], x
+    static constexpr int kKinds = 3;
+    static int kind_axis(int kind) { return kind == 1 ? 1 : 0; }
 };
 
 namespace {
@@ -1131,14 +1134,15 @@ int batch_setup(sfd_batch* b)
     long long parts = 0, sites = 0;
     for (auto& G : b->groups) {
         for (int ax = 0; ax < 2; ++ax) SFD_TRY(batch_group_variant<P>(b, G, ax, 0));
-        // norm partials come from the y sweep: room for its variant with the
-        // most tiles, so the autotuner may switch variants
+        // norm partials come from the last x sweep (one per tile of G.n[1]
+        // x lines): room for the variant with the most tiles, so the
+        // autotuner may switch variants
         long long most = 0;
-        for (int v = 0; v < n_variants<P>(G.n[1], false); ++v) {
+        for (int v = 0; v < n_variants<P>(G.n[0], true); ++v) {
             VariantInfo vi;
-            if (variant_info<P>(G.n[1], false, v, &vi)) {
+            if (variant_info<P>(G.n[0], true, v, &vi)) {
                 const int NL = vi.ts * vi.g;
-                most = std::max(most, (long long)((G.n[0] + NL - 1) / NL));
+                most = std::max(most, (long long)((G.n[1] + NL - 1) / NL));
             }
         }
         parts += most * G.count;
@@ -1153,7 +1157,7 @@ int batch_setup(sfd_batch* b)
     SFD_HIP(hipMalloc(&b->d_partial, parts * sizeof(double)));
     SFD_HIP(hipMalloc(&b->d_dpart, (size_t)ni * sfd_batch::kDiffBlocks * sizeof(double)));
     SFD_HIP(hipMalloc(&b->d_snap, (size_t)sites * sizeof(cx<S>)));
-    SFD_HIP(hipMalloc(&b->d_desc, 4 * (size_t)ni * sizeof(SweepArgs)));
+    SFD_HIP(hipMalloc(&b->d_desc, sfd_batch::kKinds * (size_t)ni * sizeof(SweepArgs)));
     b->phase.assign(2 * (size_t)ni, nullptr);
     SFD_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     for (size_t g = 0; g < b->groups.size(); ++g) {
@@ -1175,7 +1179,7 @@ int batch_bind(sfd_batch* b, void* const* psi, const void* const* V, const doubl
     using S = typename P::S;
     const int ni = b->ninst;
     std::vector<BatchInstDev> hi(ni);
-    std::vector<SweepArgs> desc(4 * (size_t)ni);
+    std::vector<SweepArgs> desc(sfd_batch::kKinds * (size_t)ni);
     long long poff = 0, soff = 0;
     const double tau = 0.5 * dt;
     for (int i = 0; i < ni; ++i) {
@@ -1186,7 +1190,7 @@ int batch_bind(sfd_batch* b, void* const* psi, const void* const* V, const doubl
         hi[i].snap = static_cast<char*>(b->d_snap) + soff * sizeof(cx<S>);
         hi[i].partial = b->d_partial + poff;
         hi[i].nsite = in.nsite;
-        hi[i].nparts = G.ntiles[1];
+        hi[i].nparts = G.ntiles[0];
         hi[i].dv = in.d[0] * in.d[1];
         // imaginary-time multipliers exp(cD xi^2 tau)/n of both axes
         for (int ax = 0; ax < 2; ++ax) {
@@ -1200,8 +1204,8 @@ int batch_bind(sfd_batch* b, void* const* psi, const void* const* V, const doubl
             if (!dev) SFD_HIP(hipMalloc(&dev, n * sizeof(cx<R>)));
             SFD_TRY(upload_sync(dev, h.data(), n * sizeof(cx<R>)));
         }
-        for (int k = 0; k < 4; ++k) {
-            const int ax = k % 2;
+        for (int k = 0; k < sfd_batch::kKinds; ++k) {
+            const int ax = sfd_batch::kind_axis(k);
             SweepArgs a;
             std::memset(&a, 0, sizeof a);
             a.psi = psi[in.user];
@@ -1212,17 +1216,18 @@ int batch_bind(sfd_batch* b, void* const* psi, const void* const* V, const doubl
             a.beta = beta[in.user];
             a.dt = dt;
             batch_geom(a, in, G, ax);
-            // x (rescale), y (+ nonlinear), x, y (+ norm partials)
+            // x (rescale), [y, nonlinear, y], x (+ norm partials): the
+            // folded Strang step of split_impl
             a.flags = kFlagImag;
             if (k == 0) a.flags |= kFlagScale;
             if (k == 1) a.flags |= kFlagNL;
-            if (k == 3) {
+            if (k == 2) {
                 a.flags |= kFlagNorm;
                 a.partial = b->d_partial + poff;
             }
             desc[(size_t)k * ni + i] = a;
         }
-        poff += G.ntiles[1];
+        poff += G.ntiles[0];
         soff += in.nsite;
     }
     SFD_TRY(upload_sync(b->d_inst, hi.data(), hi.size() * sizeof(BatchInstDev)));
@@ -1253,13 +1258,13 @@ int batch_enqueue(sfd_batch* b, int k, bool check, double tol)
                                    b->d_scale + f, b->d_snap_scale + f, b->d_done + f);
                 SFD_HIP(hipGetLastError());
             }
-            for (int kind = 0; kind < 4; ++kind) {
-                const int ax = kind % 2;
+            for (int kind = 0; kind < sfd_batch::kKinds; ++kind) {
+                const int ax = sfd_batch::kind_axis(kind);
                 const VariantInfo& vi = G.vi[ax];
                 dim3 grid((unsigned)G.ntiles[ax], 1, (unsigned)G.count);
                 dim3 block(vi.ts * vi.g * vi.n2);
                 if (!launch_batch<P>(G.n[ax], ax == 0, G.var[ax], grid, block, st,
-                                     b->d_desc + (size_t)kind * ni + f, b->d_done + f))
+                                     b->d_desc + (size_t)kind * ni + f, b->d_done + f, kind == 1))
                     return fail(SFD_ERR_SHAPE, "batch: no kernel for line length %d", G.n[ax]);
                 SFD_HIP(hipGetLastError());
             }
@@ -1285,7 +1290,8 @@ int batch_enqueue(sfd_batch* b, int k, bool check, double tol)
 void batch_drop_graph(sfd_batch* b);
 
 // Batch autotuner: for every size group and axis with more than one compiled
-// variant, time the identity round trip (tau = 0) of k_batch_sweep over the
+// variant, time the identity round trip (tau = 0) of k_batch_sweep (the
+// double round trip on y, as in a step) over the
 // group's instances on the snapshot buffer filled with unit-modulus
 // pseudo-random data (the fields are not touched), 2 warm-up launches then
 // the best of 3 batches of 8, and keep the
@@ -1351,8 +1357,9 @@ int batch_autotune_impl(sfd_batch* b, float* times)
                     const VariantInfo& vi = G.vi[ax];
                     dim3 grid((unsigned)G.ntiles[ax], 1, (unsigned)G.count);
                     dim3 block(vi.ts * vi.g * vi.n2);
+                    // the y sweep of a step is the double round trip
                     auto launch = [&]() {
-                        return launch_batch<P>(n, ax == 0, v, grid, block, b->stream, b->d_desc, b->d_done);
+                        return launch_batch<P>(n, ax == 0, v, grid, block, b->stream, b->d_desc, b->d_done, ax == 1);
                     };
                     for (int w = 0; w < 2; ++w)
                         if (!launch()) return fail(SFD_ERR_SHAPE, "batch autotune: no kernel %d for n = %d", v, n);

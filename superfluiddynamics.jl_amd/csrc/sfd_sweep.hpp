@@ -639,8 +639,9 @@ __global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
 // instance, whose descriptor (field, potential, tables, strides, flags,
 // pending scale, partial sums) sits in descs[]; instances whose done flag is
 // set are skipped.  All instances of one launch share N1, N2 and the
-// workgroup shape; grid.x*grid.y covers the largest tile count.
-template <typename S, typename R, int N1, int N2, int TS, int G>
+// workgroup shape; grid.x*grid.y covers the largest tile count.  TWICE runs
+// K(tau) N K(tau) on the lines (the folded Strang step of kRoundTripTwice).
+template <typename S, typename R, int N1, int N2, int TS, int G, bool TWICE = false>
 __global__ __launch_bounds__(TS * G * N2) void k_batch_sweep(const SweepArgs* __restrict__ descs,
                                                              const int* __restrict__ done)
 {
@@ -668,7 +669,17 @@ __global__ __launch_bounds__(TS * G * N2) void k_batch_sweep(const SweepArgs* __
     load_line<S, R, N1>(v, psi, gm);
     if (a.flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
     double nacc = 0.0;
-    round_trip_tile<S, R, N1, N2, TS, G, true>(v, vp, gm, a, lds, twA, twB, row_phase<R>(a, gm), c, g, t, nacc);
+    if constexpr (TWICE) {
+        SweepArgs a1 = a;
+        a1.flags = a.flags & ~kFlagNorm;
+        round_trip_tile<S, R, N1, N2, TS, G, true>(v, vp, gm, a1, lds, twA, twB, row_phase<R>(a, gm), c, g, t, nacc);
+        lds_barrier();  // every lane is done with the exchange buffer
+        SweepArgs a2 = a;
+        a2.flags = a.flags & ~(kFlagNL | kFlagScale);
+        round_trip_tile<S, R, N1, N2, TS, G, true>(v, vp, gm, a2, lds, twA, twB, row_phase<R>(a, gm), c, g, t, nacc);
+    } else {
+        round_trip_tile<S, R, N1, N2, TS, G, true>(v, vp, gm, a, lds, twA, twB, row_phase<R>(a, gm), c, g, t, nacc);
+    }
     if (a.flags & kFlagNorm) {
         double acc1[1] = {nacc};
         block_partials<R, Sh::NT, 1>(acc1, a.partial, a.ntiles, tile);

@@ -102,7 +102,7 @@ template <typename P, int MODE>
 struct Dispatch<P, MODE, VL<>> {
     static bool info(int, VariantInfo*) { return false; }
     static bool launch(int, dim3, dim3, hipStream_t, const SweepArgs&, bool) { return false; }
-    static bool launch_batch(int, dim3, dim3, hipStream_t, const SweepArgs*, const int*) { return false; }
+    static bool launch_batch(int, dim3, dim3, hipStream_t, const SweepArgs*, const int*, bool) { return false; }
     static int occupancy(int) { return 0; }
 };
 
@@ -142,15 +142,25 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
         }
         return Dispatch<P, MODE, VL<Vs...>>::launch(idx - 1, grid, block, s, a, persistent);
     }
+    // twice: K(tau) N K(tau) per line (strided variants only)
     static bool launch_batch(int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs* descs,
-                             const int* done)
+                             const int* done, bool twice)
     {
         if (idx == 0) {
+            if constexpr (V0::TS == 1) {
+                if (twice) return false;
+            } else {
+                if (twice) {
+                    hipLaunchKernelGGL((k_batch_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, true>), grid, block, 0,
+                                       s, descs, done);
+                    return true;
+                }
+            }
             hipLaunchKernelGGL((k_batch_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G>), grid, block, 0, s, descs,
                                done);
             return true;
         }
-        return Dispatch<P, MODE, VL<Vs...>>::launch_batch(idx - 1, grid, block, s, descs, done);
+        return Dispatch<P, MODE, VL<Vs...>>::launch_batch(idx - 1, grid, block, s, descs, done, twice);
     }
     // resident workgroups per CU of the persistent round-trip kernel
     static int occupancy(int idx)
@@ -233,18 +243,20 @@ bool variant_info(int n, bool contig, int idx, VariantInfo* out)
 }
 
 template <typename P, int N>
-bool launch_batch_n(bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs* d, const int* done)
+bool launch_batch_n(bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs* d, const int* done,
+                    bool twice)
 {
-    if (contig) return Dispatch<P, 0, typename Variants<P, N>::contig>::launch_batch(idx, grid, block, s, d, done);
-    return Dispatch<P, 0, typename Variants<P, N>::strided>::launch_batch(idx, grid, block, s, d, done);
+    if (contig)
+        return Dispatch<P, 0, typename Variants<P, N>::contig>::launch_batch(idx, grid, block, s, d, done, twice);
+    return Dispatch<P, 0, typename Variants<P, N>::strided>::launch_batch(idx, grid, block, s, d, done, twice);
 }
 
 template <typename P>
 bool launch_batch(int n, bool contig, int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs* d,
-                  const int* done)
+                  const int* done, bool twice = false)
 {
 #define SFD_CASE(N_) \
-    if (n == N_) return launch_batch_n<P, N_>(contig, idx, grid, block, s, d, done);
+    if (n == N_) return launch_batch_n<P, N_>(contig, idx, grid, block, s, d, done, twice);
     SFD_FOR_EACH_N(SFD_CASE)
 #undef SFD_CASE
     return false;
