@@ -132,8 +132,10 @@ int sfd_autotune(sfd_ctx* ctx, void* scratch, float* times_ms);
 
 /* Current launch variant of an axis: its index, the number of variants
  * compiled for this line length and precision, and (optional) its shape
- * {n1, n2, ts, g, pipelined, minimum waves per SIMD}. */
-int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* shape6);
+ * {n1, n2, ts, g, pipelined, minimum waves per SIMD, halves}: a line of
+ * halves*n1*n2 points, halves = 2 splitting it into interleaved halves
+ * joined by a radix-2 stage. */
+int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* shape7);
 /* Force a launch variant (index < count); results are identical up to
  * floating-point rounding. */
 int sfd_ctx_set_variant(sfd_ctx* ctx, int axis, int index);

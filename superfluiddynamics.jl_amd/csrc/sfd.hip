@@ -248,21 +248,26 @@ struct sfd_ctx {
 
 namespace {
 
-// Twiddles in the layout k_sweep reads, w = exp(-2 pi i / n):
-//   [0, n):   twA[s][t] = w^(s t), s < n1, t < n2
-//   [n, 2n):  twB[q][s] = w^(q s), q < n2, s < n1
+// Twiddles in the layout k_sweep reads for a line of n = h*m points split
+// as m = n1*n2 (h = 1: m = n), with w_m = exp(-2 pi i / m):
+//   [0, m):    twA[s][t] = w_m^(s t), s < n1, t < n2
+//   [m, 2m):   twB[q][s] = w_m^(q s), q < n2, s < n1
+//   [2m, 3m):  h = 2 only, the radix-2 stage's twC[k] = w_n^k, k < m
 // FP64 entries are (cos, sin) of the angle a; FP32 entries are the lifting
 // coefficients (tan(b/2), sin b) of b = a/2 with a taken in (-pi, pi].
 template <typename R>
-void fill_twiddles(std::vector<cx<R>>& w, int n, int n1, int n2)
+void fill_twiddles(std::vector<cx<R>>& w, int n, int n1, int n2, int h)
 {
-    w.resize(2 * (size_t)n);
+    const int m = n1 * n2;
+    w.resize((size_t)(h == 2 ? 3 : 2) * m);
     const long double pi = 3.14159265358979323846264338327950288L;
-    auto put = [&](size_t i, long long m) {
-        const long double a = -2.0L * pi * (long double)(m % n) / (long double)n;
+    // entry i = exp(-2 pi i e / d)
+    auto put = [&](size_t i, long long e, long long d) {
+        e %= d;
+        const long double a = -2.0L * pi * (long double)e / (long double)d;
         if constexpr (sizeof(R) == 4) {
             // lifting coefficients of the half angle b in (-pi/2, pi/2]
-            const long double b = 0.5L * (2 * (m % n) > n ? a + 2.0L * pi : a);
+            const long double b = 0.5L * (2 * e > d ? a + 2.0L * pi : a);
             w[i].re = (R)tanl(0.5L * b);
             w[i].im = (R)sinl(b);
         } else {
@@ -271,9 +276,11 @@ void fill_twiddles(std::vector<cx<R>>& w, int n, int n1, int n2)
         }
     };
     for (int s = 0; s < n1; ++s)
-        for (int t = 0; t < n2; ++t) put((size_t)s * n2 + t, (long long)t * s);
+        for (int t = 0; t < n2; ++t) put((size_t)s * n2 + t, (long long)t * s, m);
     for (int q = 0; q < n2; ++q)
-        for (int s = 0; s < n1; ++s) put((size_t)n + (size_t)q * n1 + s, (long long)s * q);
+        for (int s = 0; s < n1; ++s) put((size_t)m + (size_t)q * n1 + s, (long long)s * q, m);
+    if (h == 2)
+        for (int k = 0; k < m; ++k) put((size_t)2 * m + k, k, n);
 }
 
 int upload_sync(void* dst, const void* src, size_t bytes)
@@ -400,7 +407,7 @@ Launch sweep_geometry(const sfd_ctx* c, int axis, SweepArgs& a, bool persistent)
         const long long resident = (long long)c->pipe_occ[axis] * c->ncu;
         L.grid = dim3((unsigned)(a.ntiles < resident ? a.ntiles : resident), 1, 1);
     }
-    L.block = dim3(NL * v.n2, 1, 1);
+    L.block = dim3(NL * v.n2 * v.h, 1, 1);
     L.nblocks = (long long)L.grid.x * L.grid.y;
     return L;
 }
@@ -449,13 +456,13 @@ int set_variant(sfd_ctx* c, int axis, int idx)
         return fail(SFD_ERR_ARG, "axis %d: variant %d does not exist (%d variants)", axis, idx,
                     n_variants<P>(n, axis == 0));
     c->pipe_occ[axis] = vi.pipe ? variant_occupancy<P>(n, axis == 0, idx) : 0;
-    if (c->tw[axis] && c->vi[axis].n1 == vi.n1) {
+    if (c->tw[axis] && c->vi[axis].n1 == vi.n1 && c->vi[axis].n2 == vi.n2 && c->vi[axis].h == vi.h) {
         c->var[axis] = idx;
         c->vi[axis] = vi;
         return SFD_OK;
     }
     std::vector<cx<R>> w;
-    fill_twiddles<R>(w, n, vi.n1, vi.n2);
+    fill_twiddles<R>(w, n, vi.n1, vi.n2, vi.h);
     if (c->tw[axis]) {
         SFD_HIP(hipStreamSynchronize(c->stream));
         SFD_HIP(hipFree(c->tw[axis]));
@@ -1071,9 +1078,9 @@ int batch_group_variant(sfd_batch* b, sfd_batch::Group& G, int ax, int idx)
     VariantInfo vi;
     if (!variant_info<P>(G.n[ax], ax == 0, idx, &vi))
         return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", idx, G.n[ax]);
-    if (!G.tw[ax] || G.vi[ax].n1 != vi.n1) {
+    if (!G.tw[ax] || G.vi[ax].n1 != vi.n1 || G.vi[ax].n2 != vi.n2 || G.vi[ax].h != vi.h) {
         std::vector<cx<R>> w;
-        fill_twiddles<R>(w, G.n[ax], vi.n1, vi.n2);
+        fill_twiddles<R>(w, G.n[ax], vi.n1, vi.n2, vi.h);
         if (G.tw[ax]) {
             SFD_HIP(hipStreamSynchronize(b->stream));
             SFD_HIP(hipFree(G.tw[ax]));
@@ -1262,7 +1269,7 @@ int batch_enqueue(sfd_batch* b, int k, bool check, double tol)
                 const int ax = sfd_batch::kind_axis(kind);
                 const VariantInfo& vi = G.vi[ax];
                 dim3 grid((unsigned)G.ntiles[ax], 1, (unsigned)G.count);
-                dim3 block(vi.ts * vi.g * vi.n2);
+                dim3 block(vi.ts * vi.g * vi.n2 * vi.h);
                 if (!launch_batch<P>(G.n[ax], ax == 0, G.var[ax], grid, block, st,
                                      b->d_desc + (size_t)kind * ni + f, b->d_done + f, kind == 1))
                     return fail(SFD_ERR_SHAPE, "batch: no kernel for line length %d", G.n[ax]);
@@ -1356,7 +1363,7 @@ int batch_autotune_impl(sfd_batch* b, float* times)
                     SFD_TRY(upload_sync(b->d_desc, desc.data(), desc.size() * sizeof(SweepArgs)));
                     const VariantInfo& vi = G.vi[ax];
                     dim3 grid((unsigned)G.ntiles[ax], 1, (unsigned)G.count);
-                    dim3 block(vi.ts * vi.g * vi.n2);
+                    dim3 block(vi.ts * vi.g * vi.n2 * vi.h);
                     // the y sweep of a step is the double round trip
                     auto launch = [&]() {
                         return launch_batch<P>(n, ax == 0, v, grid, block, b->stream, b->d_desc, b->d_done, ax == 1);
@@ -1828,19 +1835,20 @@ int sfd_autotune(sfd_ctx* ctx, void* scratch, float* times_ms)
     return SFD_BY_PREC(ctx, autotune_impl, ctx, scratch, times_ms);
 }
 
-int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* shape6)
+int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* shape7)
 {
     if (!ctx || !index || !count) return fail(SFD_ERR_NULL, "sfd_ctx_get_variant: null argument");
     if (axis < 0 || axis >= ctx->ndim) return fail(SFD_ERR_ARG, "axis %d out of range", axis);
     *index = ctx->var[axis];
     *count = SFD_BY_PREC(ctx, n_variants, ctx->n[axis], axis == 0);
-    if (shape6) {
-        shape6[0] = ctx->vi[axis].n1;
-        shape6[1] = ctx->vi[axis].n2;
-        shape6[2] = ctx->vi[axis].ts;
-        shape6[3] = ctx->vi[axis].g;
-        shape6[4] = ctx->vi[axis].pipe;
-        shape6[5] = ctx->vi[axis].waves;
+    if (shape7) {
+        shape7[0] = ctx->vi[axis].n1;
+        shape7[1] = ctx->vi[axis].n2;
+        shape7[2] = ctx->vi[axis].ts;
+        shape7[3] = ctx->vi[axis].g;
+        shape7[4] = ctx->vi[axis].pipe;
+        shape7[5] = ctx->vi[axis].waves;
+        shape7[6] = ctx->vi[axis].h;
     }
     return SFD_OK;
 }

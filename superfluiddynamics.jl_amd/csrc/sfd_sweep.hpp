@@ -358,6 +358,52 @@ __device__ __forceinline__ void inverse_ba(cx<R> (&w)[N1 / N2][N2], cx<R> (&v)[N
     dft<N1, +1, R>(v);
 }
 
+// Lines of N = 2M points (H = 2 variants): the line's two lanes per
+// sub-transform lane t hold its even (h = 0) and odd (h = 1) samples,
+// y_h[m] = x[2m + h], and forward_ab / inverse_ba run the M-point transforms
+// of y_0 and y_1.  The radix-2 stage joins them across the lane pair, which
+// differ in bit log2(TS) of the thread index (same wavefront, TS < 64):
+//   X[k] = Y0[k] + w^k Y1[k],  X[k + M] = Y0[k] - w^k Y1[k],  w = e^(-2 pi i/N),
+// so lane h ends up with X[k + M h] for its k = t + N2 r + N1 k2.  twC holds
+// w^k (k < M) in the twiddle format, pre-offset by t.
+template <typename R>
+__device__ __forceinline__ cx<R> shfl_xor_cx(cx<R> v, int mask)
+{
+    return {__shfl_xor(v.re, mask), __shfl_xor(v.im, mask)};
+}
+
+template <typename R, int N1, int N2, int TS>
+__device__ __forceinline__ void radix2_fwd(cx<R> (&w)[N1 / N2][N2], const cx<R>* twC, int h)
+{
+    static_assert(TS < 64, "the lane pair must share a wavefront");
+#pragma unroll
+    for (int r = 0; r < N1 / N2; ++r)
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+            const cx<R> x = w[r][k2];
+            const cx<R> wy = tw_mul(x, twC[N2 * r + N1 * k2]);
+            const cx<R> m = h ? wy : x;  // Y0 on even lanes, w^k Y1 on odd lanes
+            const cx<R> o = shfl_xor_cx(m, TS);
+            w[r][k2] = h ? cx<R>{o.re - m.re, o.im - m.im} : cx<R>{m.re + o.re, m.im + o.im};
+        }
+}
+
+// Inverse of radix2_fwd (unscaled): lane h gets 2 Y_h[k].
+template <typename R, int N1, int N2, int TS>
+__device__ __forceinline__ void radix2_inv(cx<R> (&w)[N1 / N2][N2], const cx<R>* twC, int h)
+{
+#pragma unroll
+    for (int r = 0; r < N1 / N2; ++r)
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+            const cx<R> x = w[r][k2];
+            const cx<R> o = shfl_xor_cx(x, TS);
+            const cx<R> d = {o.re - x.re, o.im - x.im};  // X[k] - X[k + M] on odd lanes
+            const cx<R> y = tw_mulc(d, twC[N2 * r + N1 * k2]);
+            w[r][k2] = h ? y : cx<R>{x.re + o.re, x.im + o.im};
+        }
+}
+
 // The round trip of one tile in registers: forward, phase multiply,
 // inverse, then the fused epilogue (nonlinear phase, norm partial sum).
 // IMAG selects the imaginary-time form of the FP32 phase multiply and NL
@@ -366,13 +412,18 @@ __device__ __forceinline__ void inverse_ba(cx<R> (&w)[N1 / N2][N2], cx<R> (&v)[N
 // scale) to eacc[4], in FP64: [0] sum xi^2 |psi_hat|^2 over the transformed
 // line, and with kFlagPointwise in a contiguous (x) sweep [1] sum |psi|^2,
 // [2] sum V|psi|^2, [3] sum |psi|^4 (vp must then hold V).
-template <typename S, typename R, int N1, int N2, int TS, int G, bool IMAG, bool EN = false>
+// H = 2: the line is two interleaved halves joined by the radix-2 stage; g is
+// then the thread's half-line group in LDS, t its sub-transform lane and h
+// its half; twC the radix-2 twiddles (unused for H = 1).
+template <typename S, typename R, int N1, int N2, int TS, int G, bool IMAG, bool EN = false, int H = 1>
 __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1], const TileGeom& gm,
                                                 const SweepArgs& a, cx<R>* lds, const cx<R>* twA,
-                                                const cx<R>* twB, const cx<R>* phase, int c, int g, int t,
-                                                double& norm_acc, double* eacc = nullptr)
+                                                const cx<R>* twB, const cx<R>* twC, const cx<R>* phase, int c,
+                                                int g, int t, int h, double& norm_acc, double* eacc = nullptr)
 {
+    static_assert(H == 1 || H == 2, "one or two halves per line");
     constexpr int RB = N1 / N2;
+    constexpr int M = N1 * N2;  // sub-transform length; the line has H * M points
     const int flags = a.flags;
     if (flags & kFlagScale) {
         const R sc = R(*a.scale);
@@ -393,22 +444,23 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
     }
     cx<R> w[RB][N2];
     forward_ab<R, N1, N2, TS, G>(v, w, lds, twA, c, g, t);
+    if constexpr (H == 2) radix2_fwd<R, N1, N2, TS>(w, twC, h);
     if constexpr (EN) {
         // xi_k = kk * xi_1 with kk the signed frequency of k (the wavenumber
         // table's own product)
-        constexpr int N = N1 * N2;
+        constexpr int N = H * M;
         const double dk = a.xi[1];
 #pragma unroll
         for (int r = 0; r < RB; ++r)
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2) {
-                const int k = t + N2 * r + N1 * k2;
+                const int k = t + M * h + N2 * r + N1 * k2;
                 const double x = (double)(k < N / 2 ? k : k - N) * dk;
                 const double re = w[r][k2].re, im = w[r][k2].im;
                 eacc[0] += x * x * (re * re + im * im);
             }
     }
-    const cx<R>* __restrict__ ph = phase + t;
+    const cx<R>* __restrict__ ph = phase + t + M * h;
     if constexpr (sizeof(R) == 4) {
         // FP32: X m/N with X/N exact (N a power of two).  Real-time tables
         // hold the lifting coefficients (tan(b/2), sin b) of half the phase,
@@ -416,7 +468,7 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
         // by rounded (cos phi, sin phi) loses 1 - cos(phi) of the low modes
         // to rounding on every pass and drifts the norm.  Imaginary-time
         // tables hold d = (m - 1)/N (real): X/N + X d.
-        constexpr float inv_n = 1.0f / float(N1 * N2);
+        constexpr float inv_n = 1.0f / float(H * M);
 #pragma unroll
         for (int r = 0; r < RB; ++r)
 #pragma unroll
@@ -434,6 +486,7 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2) w[r][k2] = cmul(w[r][k2], ph[N2 * r + N1 * k2]);
     }
+    if constexpr (H == 2) radix2_inv<R, N1, N2, TS>(w, twC, h);
     inverse_ba<R, N1, N2, TS, G>(w, v, lds, twB, c, g, t);
     if (flags & kFlagNL) {
         if (a.save) store_line<S, R, N1>(v, static_cast<cx<S>*>(a.save), gm);
@@ -453,45 +506,62 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
     }
 }
 
-// Stages twiddles (twA | twB, 2N entries) and one phase row (N entries)
-// into LDS, then waits for every lane.
-template <typename R, int N, int NT>
+// Twiddle entries of a line of H*M points: twA | twB (M each) | twC (M, H = 2).
+template <int M, int H>
+constexpr int kTwiddles = (H == 2 ? 3 : 2) * M;
+
+// Stages the twiddles (kTwiddles<M, H> entries) and one phase row (H*M
+// entries) into LDS, then waits for every lane.
+template <typename R, int M, int H, int NT>
 __device__ __forceinline__ void stage_tables(cx<R>* tabs, const SweepArgs& a, int tid)
 {
+    constexpr int NTW = kTwiddles<M, H>;
+    constexpr int TOT = NTW + H * M;
     const cx<R>* gtw = static_cast<const cx<R>*>(a.tw);
     const cx<R>* gph = static_cast<const cx<R>*>(a.phase);
 #pragma unroll
-    for (int k = 0; k < (3 * N + NT - 1) / NT; ++k) {
+    for (int k = 0; k < (TOT + NT - 1) / NT; ++k) {
         const int i = tid + k * NT;
-        if ((3 * N) % NT == 0 || i < 3 * N) tabs[i] = i < 2 * N ? gtw[i] : gph[i - 2 * N];
+        if (TOT % NT == 0 || i < TOT) tabs[i] = i < NTW ? gtw[i] : gph[i - NTW];
     }
     lds_barrier();
 }
 
 // One launch over all tiles of an axis, one tile per workgroup; every mode.
 // W is the minimum number of wavefronts per SIMD the register allocation
-// must leave room for (1 = no constraint).
-template <typename S, typename R, int N1, int N2, int TS, int G, int MODE, int W = 1>
-__global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
+// must leave room for (1 = no constraint).  H = 2: lines of 2*N1*N2 points,
+// two lanes per sub-transform lane (see radix2_fwd).  TL: the round trips
+// read the twiddle and phase tables from LDS (one phase row: Omega = 0).
+template <typename S, typename R, int N1, int N2, int TS, int G, int MODE, int W = 1, int H = 1, bool TL = false>
+__global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
 {
-    using Sh = SweepShape<R, N1, N2, TS, G>;
-    constexpr int N = Sh::N;
+    using Sh = SweepShape<R, N1, N2, TS, H * G>;  // LDS lines: one per half-line
+    constexpr int M = Sh::N;                      // sub-transform length
+    constexpr int N = H * M;                      // line length
     constexpr int RB = Sh::RB;
-    constexpr int NL = Sh::NL;
+    constexpr int NL = TS * G;                    // lines per workgroup
     __shared__ cx<R> lds[Sh::LDS];
 
     const int tid = threadIdx.x;
     const int c = tid % TS;
-    const int t = (tid / TS) % N2;
-    const int g = tid / (TS * N2);
+    const int tl = (tid / TS) % (H * N2);  // lane within the line
+    const int t = tl / H;                  // sub-transform lane
+    const int h = tl % H;                  // half (H = 2: even or odd samples)
+    const int g = tid / (TS * N2 * H);
+    const int gh = H * g + h;              // LDS line of the half-line
     const int L = c + TS * g;
     const long long tile = (long long)blockIdx.y * gridDim.x + blockIdx.x;
-    const TileGeom gm = tile_geom<N, N2, TS, NL>(a, tile, L, t);
+    const TileGeom gm = tile_geom<N, H * N2, TS, NL>(a, tile, L, tl);
     const int flags = a.flags;
     const cx<R>* __restrict__ twA = static_cast<const cx<R>*>(a.tw) + t;
-    const cx<R>* __restrict__ twB = static_cast<const cx<R>*>(a.tw) + N + t;
+    const cx<R>* __restrict__ twB = static_cast<const cx<R>*>(a.tw) + M + t;
+    const cx<R>* __restrict__ twC = static_cast<const cx<R>*>(a.tw) + 2 * M + t;
     cx<S>* __restrict__ psi = static_cast<cx<S>*>(a.psi);
     const long long nblocks = (long long)gridDim.x * gridDim.y;
+    // spectrum position of w[r][k2] is k0 + N2 r + N1 k2; its natural-order
+    // offset from the lane's own element is (that - tl) * kstride
+    const int k0 = t + M * h;
+    const int dk = k0 - tl;
 
     cx<R> v[N1];
     R vp[N1];
@@ -503,18 +573,27 @@ __global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
             load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
         double nacc = 0.0;
         double eacc[4] = {0, 0, 0, 0};
-        if constexpr (EN) {
+        if constexpr (EN || TL) {
             // Energy sweeps read the twiddle and phase tables (one phase row:
             // Omega = 0) from LDS: with the energy sums' registers live the
             // compiler no longer batches the tables' global loads and would
-            // wait on each in turn.
-            __shared__ cx<R> tabs[3 * N];
-            stage_tables<R, N, Sh::NT>(tabs, a, tid);
-            round_trip_tile<S, R, N1, N2, TS, G, rt_imag(MODE), true>(v, vp, gm, a, lds, tabs + t, tabs + N + t,
-                                                                     tabs + 2 * N, c, g, t, nacc, eacc);
+            // wait on each in turn.  TL variants do so for the same reason
+            // (FP64 lines of 512 points); with several phase rows (Omega !=
+            // 0) they read the phase from global memory.
+            constexpr int NTW = kTwiddles<M, H>;
+            __shared__ cx<R> tabs[NTW + N];
+            stage_tables<R, M, H, Sh::NT>(tabs, a, tid);
+            if (EN || a.phase_row_stride == 0)
+                round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H>(
+                    v, vp, gm, a, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc,
+                    eacc);
+            else
+                round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H>(
+                    v, vp, gm, a, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, row_phase<R>(a, gm), c, gh, t, h,
+                    nacc, eacc);
         } else {
-            round_trip_tile<S, R, N1, N2, TS, G, rt_imag(MODE), false>(v, vp, gm, a, lds, twA, twB,
-                                                                      row_phase<R>(a, gm), c, g, t, nacc);
+            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H>(
+                v, vp, gm, a, lds, twA, twB, twC, row_phase<R>(a, gm), c, gh, t, h, nacc);
         }
         if (flags & kFlagNorm) {
             double acc1[1] = {nacc};
@@ -551,19 +630,22 @@ __global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
         // and 79 instead of 91 us at 256^3 on MI355X.  FP64 sweeps keep them
         // in global memory: the 12 KiB of staged tables would cut them from
         // 8 to 6 resident wavefronts per CU (189 instead of 168 us).
-        if constexpr (sizeof(R) == 4 || EN) {
-            __shared__ cx<R> tabs[3 * N];
-            stage_tables<R, N, Sh::NT>(tabs, a, tid);
-            round_trip_tile<S, R, N1, N2, TS, G, rt_imag(MODE), EN>(v, vp, gm, a1, lds, tabs + t, tabs + N + t,
-                                                                   tabs + 2 * N, c, g, t, nacc, eacc);
+        if constexpr (sizeof(R) == 4 || EN || TL) {
+            constexpr int NTW = kTwiddles<M, H>;
+            __shared__ cx<R> tabs[NTW + N];
+            stage_tables<R, M, H, Sh::NT>(tabs, a, tid);
+            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H>(
+                v, vp, gm, a1, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc, eacc);
             lds_barrier();  // every lane is done with the exchange buffer
-            round_trip_tile<S, R, N1, N2, TS, G, rt_imag(MODE)>(v, vp, gm, a2, lds, tabs + t, tabs + N + t,
-                                                               tabs + 2 * N, c, g, t, nacc);
+            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H>(
+                v, vp, gm, a2, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc);
         } else {
             const cx<R>* ph = row_phase<R>(a, gm);
-            round_trip_tile<S, R, N1, N2, TS, G, rt_imag(MODE)>(v, vp, gm, a1, lds, twA, twB, ph, c, g, t, nacc);
+            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H>(v, vp, gm, a1, lds, twA, twB, twC,
+                                                                              ph, c, gh, t, h, nacc);
             lds_barrier();  // every lane is done with the exchange buffer
-            round_trip_tile<S, R, N1, N2, TS, G, rt_imag(MODE)>(v, vp, gm, a2, lds, twA, twB, ph, c, g, t, nacc);
+            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H>(v, vp, gm, a2, lds, twA, twB, twC,
+                                                                              ph, c, gh, t, h, nacc);
         }
         if (flags & kFlagNorm) {
             double acc1[1] = {nacc};
@@ -577,14 +659,15 @@ __global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
     } else if constexpr (MODE == kForward) {
         cx<R> w[RB][N2];
         load_line<S, R, N1>(v, psi, gm);
-        forward_ab<R, N1, N2, TS, G>(v, w, lds, twA, c, g, t);
+        forward_ab<R, N1, N2, TS, H * G>(v, w, lds, twA, c, gh, t);
+        if constexpr (H == 2) radix2_fwd<R, N1, N2, TS>(w, twC, h);
         if (gm.active) {
             cx<S>* __restrict__ ob = static_cast<cx<S>*>(a.out) + gm.ubase;
 #pragma unroll
             for (int r = 0; r < RB; ++r)
 #pragma unroll
                 for (int k2 = 0; k2 < N2; ++k2)
-                    lane_ref(ob, (N2 * r + N1 * k2) * gm.kstride, gm.lane) = narrow<S>(w[r][k2]);
+                    lane_ref(ob, (long long)(N2 * r + N1 * k2 + dk) * gm.kstride, gm.lane) = narrow<S>(w[r][k2]);
         }
     } else if constexpr (MODE == kInverse) {
         cx<R> w[RB][N2];
@@ -593,8 +676,9 @@ __global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
         for (int r = 0; r < RB; ++r)
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2)
-                w[r][k2] = widen<R>(lane_ref(ib, (N2 * r + N1 * k2) * gm.kstride, gm.lane));
-        inverse_ba<R, N1, N2, TS, G>(w, v, lds, twB, c, g, t);
+                w[r][k2] = widen<R>(lane_ref(ib, (long long)(N2 * r + N1 * k2 + dk) * gm.kstride, gm.lane));
+        if constexpr (H == 2) radix2_inv<R, N1, N2, TS>(w, twC, h);
+        inverse_ba<R, N1, N2, TS, H * G>(w, v, lds, twB, c, gh, t);
         const R inv = R(1) / R(N);
 #pragma unroll
         for (int j = 0; j < N1; ++j) v[j] = cscale(v[j], inv);
@@ -614,8 +698,9 @@ __global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
             }
         }
         cx<R> w[RB][N2];
-        forward_ab<R, N1, N2, TS, G>(v, w, lds, twA, c, g, t);
-        const double* __restrict__ xi = a.xi + t;
+        forward_ab<R, N1, N2, TS, H * G>(v, w, lds, twA, c, gh, t);
+        if constexpr (H == 2) radix2_fwd<R, N1, N2, TS>(w, twC, h);
+        const double* __restrict__ xi = a.xi + k0;
 #pragma unroll
         for (int r = 0; r < RB; ++r)
 #pragma unroll
@@ -638,16 +723,17 @@ __global__ __launch_bounds__(TS * G * N2, W) void k_sweep(const SweepArgs a)
 // Batched round trip over independent fields: blockIdx.z selects the
 // instance, whose descriptor (field, potential, tables, strides, flags,
 // pending scale, partial sums) sits in descs[]; instances whose done flag is
-// set are skipped.  All instances of one launch share N1, N2 and the
+// set are skipped.  All instances of one launch share N1, N2, H and the
 // workgroup shape; grid.x*grid.y covers the largest tile count.  TWICE runs
 // K(tau) N K(tau) on the lines (the folded Strang step of kRoundTripTwice).
-template <typename S, typename R, int N1, int N2, int TS, int G, bool TWICE = false>
-__global__ __launch_bounds__(TS * G * N2) void k_batch_sweep(const SweepArgs* __restrict__ descs,
-                                                             const int* __restrict__ done)
+template <typename S, typename R, int N1, int N2, int TS, int G, bool TWICE = false, int H = 1, bool TL = false>
+__global__ __launch_bounds__(TS * G * N2 * H) void k_batch_sweep(const SweepArgs* __restrict__ descs,
+                                                                 const int* __restrict__ done)
 {
-    using Sh = SweepShape<R, N1, N2, TS, G>;
-    constexpr int N = Sh::N;
-    constexpr int NL = Sh::NL;
+    using Sh = SweepShape<R, N1, N2, TS, H * G>;
+    constexpr int M = Sh::N;
+    constexpr int N = H * M;
+    constexpr int NL = TS * G;
     __shared__ cx<R> lds[Sh::LDS];
     const int inst = blockIdx.z;
     if (done[inst]) return;
@@ -657,28 +743,45 @@ __global__ __launch_bounds__(TS * G * N2) void k_batch_sweep(const SweepArgs* __
 
     const int tid = threadIdx.x;
     const int c = tid % TS;
-    const int t = (tid / TS) % N2;
-    const int g = tid / (TS * N2);
+    const int tl = (tid / TS) % (H * N2);
+    const int t = tl / H;
+    const int h = tl % H;
+    const int g = tid / (TS * N2 * H);
+    const int gh = H * g + h;
     const int L = c + TS * g;
-    const TileGeom gm = tile_geom<N, N2, TS, NL>(a, tile, L, t);
+    const TileGeom gm = tile_geom<N, H * N2, TS, NL>(a, tile, L, tl);
     const cx<R>* __restrict__ twA = static_cast<const cx<R>*>(a.tw) + t;
-    const cx<R>* __restrict__ twB = static_cast<const cx<R>*>(a.tw) + N + t;
+    const cx<R>* __restrict__ twB = static_cast<const cx<R>*>(a.tw) + M + t;
+    const cx<R>* __restrict__ twC = static_cast<const cx<R>*>(a.tw) + 2 * M + t;
     cx<S>* __restrict__ psi = static_cast<cx<S>*>(a.psi);
     cx<R> v[N1];
     R vp[N1];
     load_line<S, R, N1>(v, psi, gm);
     if (a.flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
     double nacc = 0.0;
+    const cx<R>* ph = row_phase<R>(a, gm);
+    if constexpr (TL) {  // tables from LDS (batch phase tables have one row)
+        constexpr int NTW = kTwiddles<M, H>;
+        __shared__ cx<R> tabs[NTW + N];
+        stage_tables<R, M, H, Sh::NT>(tabs, a, tid);
+        twA = tabs + t;
+        twB = tabs + M + t;
+        twC = tabs + 2 * M + t;
+        ph = tabs + NTW;
+    }
     if constexpr (TWICE) {
         SweepArgs a1 = a;
         a1.flags = a.flags & ~kFlagNorm;
-        round_trip_tile<S, R, N1, N2, TS, G, true>(v, vp, gm, a1, lds, twA, twB, row_phase<R>(a, gm), c, g, t, nacc);
+        round_trip_tile<S, R, N1, N2, TS, H * G, true, false, H>(v, vp, gm, a1, lds, twA, twB, twC, ph, c, gh, t,
+                                                                 h, nacc);
         lds_barrier();  // every lane is done with the exchange buffer
         SweepArgs a2 = a;
         a2.flags = a.flags & ~(kFlagNL | kFlagScale);
-        round_trip_tile<S, R, N1, N2, TS, G, true>(v, vp, gm, a2, lds, twA, twB, row_phase<R>(a, gm), c, g, t, nacc);
+        round_trip_tile<S, R, N1, N2, TS, H * G, true, false, H>(v, vp, gm, a2, lds, twA, twB, twC, ph, c, gh, t,
+                                                                 h, nacc);
     } else {
-        round_trip_tile<S, R, N1, N2, TS, G, true>(v, vp, gm, a, lds, twA, twB, row_phase<R>(a, gm), c, g, t, nacc);
+        round_trip_tile<S, R, N1, N2, TS, H * G, true, false, H>(v, vp, gm, a, lds, twA, twB, twC, ph, c, gh, t, h,
+                                                                 nacc);
     }
     if (a.flags & kFlagNorm) {
         double acc1[1] = {nacc};
@@ -743,7 +846,7 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
             gn = tile_geom<N, N2, TS, NL>(a, next, L, t);
             load_line<S, R, N1>(vn, psi, gn);
         }
-        round_trip_tile<S, R, N1, N2, TS, G, IMAG>(v, vp, gm, a, lds, twA, twB, ph, c, g, t, nacc);
+        round_trip_tile<S, R, N1, N2, TS, G, IMAG>(v, vp, gm, a, lds, twA, twB, nullptr, ph, c, g, t, 0, nacc);
         store_line_all<S, R, N1>(v, psi, gm);
         if (!more) break;
         // the next tile's loads were issued before these N1 stores: wait for

@@ -10,15 +10,21 @@
 namespace sfd {
 
 struct VariantInfo {
-    int n1, n2, ts, g, pipe, waves;
+    int n1, n2, ts, g, pipe, waves, h;
 };
 
 // pipe = 1: the round trip runs as the persistent, software-pipelined
 // k_sweep_pipe (the other modes use k_sweep with the same shape).
 // W: minimum wavefronts per SIMD the register allocation must allow.
-template <int N1_, int N2_, int TS_, int G_, int P_ = 0, int W_ = 1>
+// H = 2: lines of 2*N1*N2 points as two interleaved halves of N1*N2 points
+// joined by a radix-2 stage across lane pairs (H*N2 lanes per line), which
+// keeps N1 = 16 points per lane for 512-point lines.
+// T = 1: the round trips read their tables from LDS (k_sweep TL).
+template <int N1_, int N2_, int TS_, int G_, int P_ = 0, int W_ = 1, int H_ = 1, int T_ = 0>
 struct V {
-    static constexpr int N1 = N1_, N2 = N2_, TS = TS_, G = G_, P = P_, W = W_;
+    static constexpr int N1 = N1_, N2 = N2_, TS = TS_, G = G_, P = P_, W = W_, H = H_;
+    static constexpr bool T = T_ != 0;
+    static_assert(P_ == 0 || H_ == 1, "the pipelined kernel takes whole lines");
 };
 template <typename... Vs>
 struct VL {
@@ -71,7 +77,8 @@ SFD_VARIANTS(PF32, 128, (VL<V<16, 8, 1, 32>, V<16, 8, 1, 16>, V<32, 4, 1, 32>, V
              (VL<V<16, 8, 16, 1>, V<16, 8, 32, 1>, V<32, 4, 32, 1>, V<16, 8, 16, 1, 1>>))
 SFD_VARIANTS(PF32, 256, (VL<V<16, 16, 1, 16>, V<16, 16, 1, 8>, V<32, 8, 1, 16>, V<16, 16, 1, 16, 1>, V<16, 16, 1, 8, 1>>),
              (VL<V<16, 16, 16, 1>, V<16, 16, 32, 1>, V<32, 8, 16, 1>, V<16, 16, 16, 1, 1>, V<16, 16, 8, 1, 1>>))
-SFD_VARIANTS(PF32, 512, (VL<V<32, 16, 1, 8>, V<32, 16, 1, 4>>), (VL<V<32, 16, 16, 1>, V<32, 16, 8, 1>>))
+SFD_VARIANTS(PF32, 512, (VL<V<32, 16, 1, 8>, V<32, 16, 1, 4>, V<16, 16, 1, 8, 0, 1, 2>, V<16, 16, 1, 4, 0, 1, 2>>),
+             (VL<V<32, 16, 16, 1>, V<32, 16, 8, 1>, V<16, 16, 16, 1, 0, 1, 2>, V<16, 16, 8, 1, 0, 1, 2>>))
 SFD_VARIANTS(PF32, 1024, (VL<V<32, 32, 1, 4>, V<32, 32, 1, 2>>), (VL<V<32, 32, 8, 1>, V<32, 32, 4, 1>>))
 
 // ---------------------------------------------------------------- FP64
@@ -85,7 +92,11 @@ SFD_VARIANTS(PF64, 128, (VL<V<16, 8, 1, 16>, V<16, 8, 1, 8>, V<32, 4, 1, 16>>),
              (VL<V<16, 8, 16, 1>, V<16, 8, 32, 1>, V<32, 4, 16, 1>>))
 SFD_VARIANTS(PF64, 256, (VL<V<16, 16, 1, 8>, V<16, 16, 1, 4>, V<16, 16, 1, 8, 1>>),
              (VL<V<16, 16, 16, 1>, V<16, 16, 8, 1>, V<16, 16, 8, 1, 1>>))
-SFD_VARIANTS(PF64, 512, (VL<V<32, 16, 1, 4>>), (VL<V<32, 16, 8, 1>, V<32, 16, 4, 1>>))
+SFD_VARIANTS(PF64, 512,
+             (VL<V<16, 16, 1, 4, 0, 1, 2>, V<32, 16, 1, 4>, V<16, 16, 1, 8, 0, 1, 2>, V<16, 16, 1, 2, 0, 1, 2, 1>,
+                 V<16, 16, 1, 4, 0, 1, 2, 1>>),
+             (VL<V<16, 16, 8, 1, 0, 1, 2>, V<32, 16, 8, 1>, V<32, 16, 4, 1>, V<16, 16, 16, 1, 0, 1, 2>,
+                 V<16, 16, 4, 1, 0, 1, 2, 1>, V<16, 16, 8, 1, 0, 1, 2, 1>>))
 SFD_VARIANTS(PF64, 1024, (VL<V<32, 32, 1, 2>>), (VL<V<32, 32, 4, 1>, V<32, 32, 2, 1>>))
 
 #undef SFD_VARIANTS
@@ -113,7 +124,7 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
     static bool info(int idx, VariantInfo* out)
     {
         if (idx == 0) {
-            *out = {V0::N1, V0::N2, V0::TS, V0::G, V0::P, V0::W};
+            *out = {V0::N1, V0::N2, V0::TS, V0::G, V0::P, V0::W, V0::H};
             return true;
         }
         return Dispatch<P, MODE, VL<Vs...>>::info(idx - 1, out);
@@ -135,8 +146,8 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
             if constexpr (rt_twice(MODE) && V0::TS == 1) {
                 return false;  // the double round trip runs on a strided (y/z) axis only
             } else {
-                hipLaunchKernelGGL((k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W>), grid, block, 0, s,
-                                   a);
+                hipLaunchKernelGGL((k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W, V0::H, V0::T>), grid,
+                                   block, 0, s, a);
                 return true;
             }
         }
@@ -151,13 +162,13 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
                 if (twice) return false;
             } else {
                 if (twice) {
-                    hipLaunchKernelGGL((k_batch_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, true>), grid, block, 0,
-                                       s, descs, done);
+                    hipLaunchKernelGGL((k_batch_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, true, V0::H, V0::T>),
+                                       grid, block, 0, s, descs, done);
                     return true;
                 }
             }
-            hipLaunchKernelGGL((k_batch_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G>), grid, block, 0, s, descs,
-                               done);
+            hipLaunchKernelGGL((k_batch_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, false, V0::H, V0::T>), grid,
+                               block, 0, s, descs, done);
             return true;
         }
         return Dispatch<P, MODE, VL<Vs...>>::launch_batch(idx - 1, grid, block, s, descs, done, twice);

@@ -153,9 +153,10 @@ class _Ctx:
         return {a: [t for t in times[a * 8:(a + 1) * 8] if t >= 0] for a in range(3) if times[a * 8] >= 0}
 
     def variant(self, axis: int):
-        """(index, count, (n1, n2, ts, g, pipelined, min waves/SIMD)) of the sweep variant on axis."""
+        """(index, count, (n1, n2, ts, g, pipelined, min waves/SIMD, halves)) of the
+        sweep variant on axis."""
         idx, cnt = ctypes.c_int(), ctypes.c_int()
-        shp = (ctypes.c_int * 6)()
+        shp = (ctypes.c_int * 7)()
         L.check(self.lib.sfd_ctx_get_variant(self.h, axis, ctypes.byref(idx), ctypes.byref(cnt), shp))
         return idx.value, cnt.value, tuple(shp)
 

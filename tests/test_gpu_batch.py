@@ -6,7 +6,7 @@ from helpers import PRECS, TOLS, gpu_run, random_potential, rel_l2, smooth_state
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(32, 32), (64, 32), (32, 32), (128, 64), (16, 256), (64, 32)]
+SHAPES = [(32, 32), (64, 32), (32, 32), (128, 64), (16, 256), (64, 32), (512, 16), (16, 512)]
 
 
 def _make(gpu, prec, seed=0):

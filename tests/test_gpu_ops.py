@@ -18,7 +18,7 @@ def _field(sfd, psi, bounds, FT=np.float64):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("shape", [(32, 16), (256, 64), (16, 32, 8), (64, 128, 32)])
+@pytest.mark.parametrize("shape", [(32, 16), (256, 64), (16, 32, 8), (64, 128, 32), (512, 8), (8, 16, 512)])
 def test_fft_axis_matches_numpy(gpu, shape, prec):
     """mul_d! is the unnormalised forward FFT along d, ldiv_d! the normalised
     inverse (Plans.jl:1132-1250); both in pen_x layout here."""
@@ -63,7 +63,8 @@ def test_norm_normalize(gpu):
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("case", [((64, 32), ((-6, 6), (-3, 3)), 0.7),
-                                  ((32, 16, 64), ((-4, 4), (-3, 3), (-5, 5)), 0.5)], ids=["2d", "3d"])
+                                  ((32, 16, 64), ((-4, 4), (-3, 3), (-5, 5)), 0.5),
+                                  ((512, 16), ((-8, 8), (-2, 2)), 0.3)], ids=["2d", "3d", "2d-512"])
 def test_energy_matches_oracle(gpu, orc, case, prec):
     shape, bounds, Om = case
     FT = np.float64 if prec == "f64" else np.float32

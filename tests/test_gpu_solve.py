@@ -15,6 +15,7 @@ pytestmark = pytest.mark.gpu
 CASES = {
     "2d": ((64, 32), ((-6, 6), (-3, 3))),
     "3d": ((32, 16, 64), ((-4, 4), (-3, 3), (-5, 5))),
+    "3d-512": ((512, 8, 512), ((-8, 8), (-2, 2), (-8, 8))),  # 512-point lines on x and z (halved lines)
 }
 
 
