@@ -170,7 +170,8 @@ def solve_driver(n=256, iters=20, prec="f32"):
 def traffic_driver(n=256, launches=5):
     """Launches for the rocprofv3 --pmc runs: `launches` round trips of each
     axis of the 256³ FP32 workload under the autotuned variants (x: the
-    contiguous kernel, then y and z: the strided kernel), then `launches`
+    contiguous kernel, then y and z: the strided kernel), `launches` Split2
+    steps (whose [z N z] sweep is the double round trip), then `launches`
     in-place passes of a plain elementwise
     kernel over the same 128 MiB (torch mul_), the calibration whose
     algorithmic traffic is exactly one read and one write of the field."""
@@ -192,6 +193,10 @@ def traffic_driver(n=256, launches=5):
             assert rc == 0, lib.sfd_last_error()
         torch.cuda.synchronize()
         print(f"[traffic] axis {ax}: {launches} launches, variant {ctx.variant(ax)[2]}", flush=True)
+    rc = lib.sfd_split_step(ctx.h, psi, None, mp["coeffD"], mp["beta"], 0.0, mp["dt"], launches, 2, 0)
+    assert rc == 0, lib.sfd_last_error()
+    torch.cuda.synchronize()
+    print(f"[traffic] {launches} Split2 steps", flush=True)
     flat = torch.view_as_real(f.ϕ)
     for _ in range(launches):
         flat.mul_(1.0)
