@@ -332,7 +332,7 @@ def main(argv=None):
         "f32_over_f64_speedup": out["f64"]["ms_per_step"] / out["f32"]["ms_per_step"],
         "f32x_over_f64_speedup": out["f64"]["ms_per_step"] / out["f32x"]["ms_per_step"],
         "f32_step_hbm_frac": out["f32"]["step_hbm_frac"],
-        "f32_step_hbm_frac_note": "the step moves 5 passes x 16 B = 80 B per site-update (x, y, [z N z], y, x); "
+        "f32_step_hbm_frac_note": "the step moves 5 passes x 16 B = 80 B per site-update (x, y, [z N z], x, y); "
                                   "priced at the 6-pass 96 B of the reference's structure it would read "
                                   f"{out['f32']['step_hbm_frac_6_pass_bytes']:.3f}",
         "f32_sweep_ms": out["f32"]["sweep_ms"],

@@ -584,13 +584,14 @@ template <typename P>
 // z sums are taken after the x (and y) flow, which leaves them unchanged
 // only in real time with Omega = 0 (the caller's condition).
 //
-// Strang steps with Omega = 0 run the axes as x, y, [z N z],
-// y, x (fold): all half-step flows are Fourier multipliers along different
-// axes and commute, so K(tau) = K_z K_y K_x may apply its axes in either
-// order, and the two flows of the innermost axis meet around the nonlinear
-// phase in one double round-trip sweep (2D - 1 sweeps per step instead of
-// 2D).  `fold = false` keeps the x, y, z, N, x, y, z order of
-// splitting.jl:149-154.
+// Strang steps run the axes as x, y, [z N z], x, y (fold): the innermost
+// axis's flow commutes with the others (its multiplier depends on its own
+// wavenumber only; with rotation the x and y flows also depend on y and x,
+// so this holds for z in 3D, and for y in 2D only when Omega = 0), so the
+// second half step K_z K_y K_x may apply z first, and the two z flows meet
+// around the nonlinear phase in one double round-trip sweep (2D - 1 sweeps
+// per step instead of 2D).  `fold = false` keeps the x, y, z, N, x, y, z
+// order of splitting.jl:149-154.
 int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt,
                int64_t nsteps, int order, int mode, double* ehist = nullptr, bool fold = true)
 {
@@ -603,8 +604,9 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
     const double tau = order == 2 ? dt * 0.5 : dt;
     const int nlflags = kFlagNL | (imag ? kFlagImag : 0);
     // Reduction workspace for the per-step renormalisation (its last sweep
-    // is on x when folded, else on the last axis).
-    if (imag) SFD_TRY(ensure_partial(c, 5 * std::max(sweep_nblocks(c, 0), sweep_nblocks(c, D - 1))));
+    // is on axis D - 2 when folded, else on the last axis).
+    if (imag)
+        SFD_TRY(ensure_partial(c, 5 * std::max(sweep_nblocks(c, D >= 2 ? D - 2 : 0), sweep_nblocks(c, D - 1))));
     // Energy partials: one region per axis, all finalized after the step's
     // first half in one launch.
     double* epart[3] = {nullptr, nullptr, nullptr};
@@ -618,7 +620,7 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
         for (int ax = 0; ax < D; ++ax) epart[ax] = c->epartial + off[ax];
     }
     bool pending = false;
-    fold = fold && order == 2 && Om == 0.0 && D >= 2;
+    fold = fold && order == 2 && D >= 2 && (Om == 0.0 || D == 3);
     for (int64_t s = 0; s < nsteps && fold; ++s) {
         FinalizeSets fs{};
         auto record = [&](int ax) {
@@ -639,8 +641,9 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
             hipLaunchKernelGGL(k_finalize_sets, dim3(fs.rows()), dim3(kFinalizeThreads), 0, c->stream, fs);
             SFD_HIP(hipGetLastError());
         }
-        for (int ax = D - 2; ax >= 0; --ax)
-            SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, (ax == 0 && imag) ? kFlagNorm : 0, beta, dt));
+        for (int ax = 0; ax < D - 1; ++ax)
+            SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, (ax == D - 2 && imag) ? kFlagNorm : 0, beta,
+                                  dt));
         if (imag) {
             launch_finalize(c->stream, c->partial, c->last_nblocks, 1, c->scal + 1, c->dvol(), c->scal);
             SFD_HIP(hipGetLastError());
