@@ -11,13 +11,15 @@
 # Julia is not installed in the build container, so this file is not exercised
 # by the test suite; the same entry points are exercised through ctypes
 # (superfluiddynamics.jl_amd/sfdhip, tests/).  Dependencies of the drop-in:
-# SuperfluidDynamics, AMDGPU, ChainRulesCore.
+# SuperfluidDynamics, AMDGPU, AbstractFFTs, PencilArrays, ChainRulesCore.
 module SuperfluidDynamicsHIP
 
 using SuperfluidDynamics
 using SuperfluidDynamics: AbstractField, Field, Grid, GrossPitaevskiiParameters,
                           NumModelSplit1, NumModelSplit2, PotentialZero
 using AMDGPU: ROCArray, stream
+using AbstractFFTs: fftfreq
+using PencilArrays: PencilArray
 import ChainRulesCore
 
 const libsfd = normpath(joinpath(@__DIR__, "..", "sfdhip", "libsfdhip.so"))
@@ -107,6 +109,56 @@ for (d, ax) in ((:x, 0), (:y, 1), (:z, 2))
     @eval $ldv(out::HipField, f::HipField) =
         check(ccall((:sfd_fft_axis, libsfd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint),
                     context(f).handle, devptr(f.ϕ), devptr(out.ϕ), $ax, SFD_BACKWARD))
+end
+
+# Plan(f; t=FFTPlan()) for fields in HIP memory (Plans.jl:821-854,
+# 958-999): the AbstractFFTs plug of the other FFT-based models
+# (NumModelGPRK, NumModelExternalVelocity, computeDerivatives!).  Every
+# buffer stays in the pen_x layout: mul_d!/ldiv_d! call sfd_fft_axis along
+# axis d with the spectrum left where the samples were (no transposes), and
+# grid_d returns the coordinates and wavenumbers broadcast in that layout,
+# so the reference's multiplier broadcasts line up with it.  The buffer
+# accessors (ϕx_hat, a_tmpx, a_tmp2x, ...; Plans.jl:1078-1124) come from
+# AbstractFFTPlan through the datax/datay/dataz fields.
+struct HipFFTPlan{N,F,V,B} <: SuperfluidDynamics.AbstractFFTPlan{N}
+    f::F
+    x::V
+    ξ::V
+    datax::Vector{B}
+    datay::Vector{B}
+    dataz::Vector{B}
+end
+
+function SuperfluidDynamics.Plan(f::HipField; t=SuperfluidDynamics.FFTPlan())
+    t isa SuperfluidDynamics.FFTPlan || error("HIP fields take FFTPlan only")
+    g = f.g
+    nd = ndims(f.ϕ)
+    ns = size(f.ϕ)
+    Δs = (g.Δx, g.Δy, g.Δz)[1:nd]
+    xs = (g.x, g.y, g.z)[1:nd]
+    bc(v, a) = reshape(ROCArray(collect(v)), ntuple(i -> i == a ? ns[a] : 1, nd))
+    x = Tuple(bc(xs[a], a) for a in 1:nd)
+    ξ = Tuple(bc(fftfreq(ns[a], 2π / Δs[a]), a) for a in 1:nd)
+    buf() = [similar(f.ϕ) for _ in 1:3]
+    return HipFFTPlan{1,typeof(f),typeof(x),typeof(f.ϕ)}(f, x, ξ, buf(), buf(), buf())
+end
+
+function hip_fft!(p::HipFFTPlan, out, src, ax::Integer, dir::Cint)
+    check(ccall((:sfd_fft_axis, libsfd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint),
+                context(p.f).handle, devptr(src), devptr(out), ax, dir))
+    return nothing
+end
+
+for (d, ax) in ((:x, 0), (:y, 1), (:z, 2))
+    mul = Symbol(:mul_, d, :!)
+    ldv = Symbol(:ldiv_, d, :!)
+    grd = Symbol(:grid_, d)
+    # PencilArray arguments, as the reference's methods take (Plans.jl:1132-1250)
+    @eval SuperfluidDynamics.$mul(out::PencilArray, p::HipFFTPlan, src::PencilArray) =
+        hip_fft!(p, out, src, $ax, SFD_FORWARD)
+    @eval SuperfluidDynamics.$ldv(out::PencilArray, p::HipFFTPlan, src::PencilArray) =
+        hip_fft!(p, out, src, $ax, SFD_BACKWARD)
+    @eval SuperfluidDynamics.$grd(p::HipFFTPlan) = (p.x..., p.ξ...)
 end
 
 # Imaginary-time Strang steps with FP64 renormalisation after every step.
