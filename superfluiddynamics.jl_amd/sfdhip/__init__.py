@@ -7,6 +7,7 @@ from .api import (  # noqa: F401
     NumModelSplit1, NumModelSplit2, NumModelSplitImag, timeStep, energy, energy_all, solve, steps_with_energy,
     solveLapRot, solveNL, BatchImagSolver, adjoint_gradient, FidelityLoss, fidelity_loss, project, inner,
     mul_x, mul_y, mul_z, ldiv_x, ldiv_y, ldiv_z, grid_x, grid_y, grid_z, linrange_coords, fftfreq,
+    Plan, GradientField, computeDerivatives,
 )
 from .io import WriterSave, write, read, CheckpointError  # noqa: F401
 

@@ -508,6 +508,60 @@ grid_y = grid_x
 grid_z = grid_x
 
 
+class Plan:
+    """Plan(f; t=FFTPlan()) (Plans.jl:821-854, 958-999) on a HIP field: the
+    scratch buffers of the FFT plan (ϕx_hat, a_tmpx, a_tmp2x and the y/z ones,
+    Plans.jl:1078-1100), all in the pen_x layout, with mul_d / ldiv_d through
+    sfd_fft_axis.  The plug of the reference's other FFT-based models."""
+
+    def __init__(self, f: Field):
+        self.f = f
+        for d in "xyz"[:f.g.ndims]:
+            for name in ("ϕ{}_hat", "a_tmp{}", "a_tmp2{}"):
+                setattr(self, name.format(d), torch.empty_like(f.ϕ))
+        rdt = torch.float32 if f.g.FT is np.float32 else torch.float64
+        host = grid_x(f)
+        self.grid = tuple(torch.as_tensor(np.ascontiguousarray(a), dtype=rdt, device=f.ϕ.device) for a in host)
+
+    def mul(self, out, src, axis: int):
+        _fft(self.f, out, src, axis, L.SFD_FORWARD)
+
+    def ldiv(self, out, src, axis: int):
+        _fft(self.f, out, src, axis, L.SFD_BACKWARD)
+
+
+class GradientField:
+    """GradientField(f; rotation=true) (Fields.jl): dx, dy[, dz], ddx, ddy[, ddz],
+    rx, ry on the device."""
+
+    def __init__(self, f: Field):
+        for name in ("dx", "dy", "dz", "ddx", "ddy", "ddz", "rx", "ry")[:]:
+            if name[-1] == "z" and f.g.ndims < 3:
+                continue
+            setattr(self, name, torch.empty_like(f.ϕ))
+
+
+def computeDerivatives(gf: GradientField, p: Plan, ϕt) -> None:
+    """computeDerivatives!(gf::GradientRot*, p, ϕt) (derivatives.jl:27-54, 90-126):
+    per axis d, ϕd_hat = mul_d(ϕt); dd = ldiv_d(i ξd ϕd_hat), ddd = ldiv_d(-ξd² ϕd_hat);
+    rx = ldiv_x(i y ξx ϕx_hat), ry = ldiv_y(-i x ξy ϕy_hat)."""
+    nd = p.f.g.ndims
+    co, xi = p.grid[:nd], p.grid[nd:]
+    for a, d in enumerate("xyz"[:nd]):
+        hat, tmp = getattr(p, f"ϕ{d}_hat"), getattr(p, f"a_tmp2{d}")
+        p.mul(hat, ϕt, a)
+        torch.mul(hat, 1j * xi[a], out=tmp)
+        p.ldiv(getattr(gf, f"d{d}"), tmp, a)
+        torch.mul(hat, -(xi[a] ** 2), out=tmp)
+        p.ldiv(getattr(gf, f"dd{d}"), tmp, a)
+        if a == 0 and nd >= 2:
+            torch.mul(hat, 1j * co[1] * xi[0], out=tmp)
+            p.ldiv(gf.rx, tmp, 0)
+        if a == 1:
+            torch.mul(hat, -1j * co[0] * xi[1], out=tmp)
+            p.ldiv(gf.ry, tmp, 1)
+
+
 # --------------------------------------------------- batched ground states
 class BatchImagSolver:
     """Many independent 2D imaginary-time ground-state solves advanced
