@@ -531,7 +531,8 @@ __device__ __forceinline__ void stage_tables(cx<R>* tabs, const SweepArgs& a, in
 // W is the minimum number of wavefronts per SIMD the register allocation
 // must leave room for (1 = no constraint).  H = 2: lines of 2*N1*N2 points,
 // two lanes per sub-transform lane (see radix2_fwd).  TL: the round trips
-// read the twiddle and phase tables from LDS (one phase row: Omega = 0).
+// read the twiddle tables from LDS, and the phase table too when it has one
+// row (no rotation term on the axis).
 template <typename S, typename R, int N1, int N2, int TS, int G, int MODE, int W = 1, int H = 1, bool TL = false>
 __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
 {
@@ -609,10 +610,11 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
         store_line<S, R, N1>(v, psi, gm);
     } else if constexpr (rt_twice(MODE)) {
         // K_d(tau) N(dt) K_d(tau) on the lines of one axis: the innermost
-        // axis of a Strang step whose half-step flows commute (Omega = 0),
-        // so that the two half steps meet at this axis around the
-        // nonlinear phase.  One read and one write of the field for both
-        // round trips; the phase table is the same for both.
+        // axis of a Strang step, whose flow commutes with the other axes'
+        // flows (z in 3D; y in 2D when Omega = 0), so that the two half
+        // steps meet at this axis around the nonlinear phase.  One read and
+        // one write of the field for both round trips; the phase table (one
+        // row: no rotation term on this axis) is the same for both.
         // kRoundTripTwiceEnergy: the first round trip also sums the energy
         // of the loaded state (spectral sum only: this is never the x axis).
         constexpr bool EN = rt_energy(MODE);
@@ -625,8 +627,8 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
         SweepArgs a2 = a;
         a2.flags = flags & ~(kFlagNL | kFlagScale);
         a2.save = nullptr;
-        // FP32 (and energy) sweeps read the tables (one phase row: Omega =
-        // 0) from LDS, staged once per workgroup: 104 instead of 126 VGPRs,
+        // FP32 (and energy) sweeps read the tables from LDS, staged once
+        // per workgroup: 104 instead of 126 VGPRs,
         // and 79 instead of 91 us at 256^3 on MI355X.  FP64 sweeps keep them
         // in global memory: the 12 KiB of staged tables would cut them from
         // 8 to 6 resident wavefronts per CU (189 instead of 168 us).
