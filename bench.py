@@ -205,6 +205,49 @@ def traffic_driver(n=256, launches=5):
     return 0
 
 
+def adjoint_driver(prec="f64", nsteps=800, reps=3):
+    """The 256² control problem (V = ½|x|² + Σ_j a_j exp(-|x - c_j|²/0.5) on an
+    8×8 lattice, β = 500, Δt = 1e-3, the Thomas-Fermi state as ψ0 and its
+    shift by +1.5 in x as the target): wall time of `nsteps` Split2 steps,
+    and of sfd_adjoint over the same steps (loss, dL/dV, dL/dβ, √N
+    checkpoints), best of `reps` each."""
+    import numpy as np
+    import torch
+    import sfdhip as sfd
+    from sfdhip.workloads import control_problem, shift_x, thomas_fermi_2d
+    basis, a, V0, bounds = control_problem()
+    n = basis.shape[-1]
+    V = np.ascontiguousarray(V0 + np.tensordot(a, basis, axes=1))
+    psi0, _, _ = thomas_fermi_2d(n=n, L=8.0, beta=500.0, seed=0)
+    FT, compute = PRECISIONS[prec]
+    g = sfd.Grid((n, n), bounds, FT=np.dtype(FT), compute=compute)
+    f = sfd.Field(g)
+    f.autotune()
+    tgt = sfd.Field(g)
+    tgt.set(np.ascontiguousarray(shift_x(psi0, 1.5, 8.0)))
+    p = sfd.GrossPitaevskiiParameters(pot=sfd.PotentialArray(f, V), coeffΔ=-0.5, β=500.0)
+    model = sfd.NumModelSplit2(f, p, 1e-3, nsteps, nsteps)
+    fwd, adj = [], []
+    for _ in range(reps):
+        f.set(psi0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        model.steps(nsteps)
+        torch.cuda.synchronize()
+        fwd.append(time.perf_counter() - t0)
+        f.set(psi0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss, gV, gb = sfd.adjoint_gradient(model, tgt, nsteps)
+        torch.cuda.synchronize()
+        adj.append(time.perf_counter() - t0)
+    r = {"adjoint_driver": prec, "n": n, "nsteps": nsteps, "forward_ms": 1e3 * min(fwd),
+         "adjoint_ms": 1e3 * min(adj), "adjoint_over_forward": min(adj) / min(fwd), "loss": loss,
+         "dL_dbeta": gb, "dL_dV_l2": float(torch.linalg.vector_norm(gV))}
+    print(json.dumps(r), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,7 +264,11 @@ def main(argv=None):
                     help="only launch the kernels profiled by the rocprofv3 --pmc traffic runs")
     ap.add_argument("--solve-driver", choices=["f32", "f32x", "f64"],
                     help="only run solve!'s loop (separate and fused energies) for a kernel trace")
+    ap.add_argument("--adjoint-driver", choices=["f32", "f32x", "f64"],
+                    help="only time Split2 steps and sfd_adjoint on the 256² control problem")
     args = ap.parse_args(argv)
+    if args.adjoint_driver:
+        return adjoint_driver(prec=args.adjoint_driver)
     if args.traffic_driver:
         return traffic_driver(args.n)
     if args.solve_driver:
