@@ -5,10 +5,15 @@ Workload: periodic box [0, 2π)³, 256³ sites, V = 0, coeffΔ = -0.5,
 ξ = 2Δ, β = 1/ξ², Δt = 5e-4; ψ0 = e^{iθ} with θ a seeded Gaussian random
 field on 1 <= |k| <= 8 (modal variance ∝ |k|^-2, rms 1).  A step is one full
 Strang step (x, y, z round trips over Δt/2, the nonlinear phase over Δt,
-x, y, z again) of the whole field, which stays resident in HBM.  FP32 is the
-headline ``value``; FP64 is measured side by side.
+x, y, z again) of the whole field, which stays resident in HBM; with Ω = 0
+it runs as x, y, [z N z], y, x.  FP32 is the headline ``value``; FP32
+storage with FP64 arithmetic and FP64 are measured side by side.
 
 Prints one JSON line.  ``python bench.py --gpus N --steps K --warmup W``.
+Drivers for profiling runs, each printing its own line: ``--traffic-driver``
+(the launches of the rocprofv3 --pmc runs), ``--solve-driver PREC`` (the
+solve! loop), ``--adjoint-driver PREC`` (Split2 steps and sfd_adjoint on
+the 256² control problem).
 """
 from __future__ import annotations
 
