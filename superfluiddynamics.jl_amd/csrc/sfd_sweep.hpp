@@ -336,7 +336,11 @@ __device__ __forceinline__ void forward_ab(cx<R> (&v)[N1], cx<R> (&w)[N1 / N2][N
 // Inverse transform (unscaled): inverse phase B, conjugate twiddles, LDS
 // exchange, inverse phase A; on return v[j] = N * x[N2 j + t].  Each lane
 // only rewrites the LDS slots it read in forward_ab, so no barrier is needed
-// before the writes.
+// before the writes.  Likewise the next forward_ab on the same buffer writes
+// exactly the slots this lane reads at the end of inverse_ba, so round trips
+// may follow each other (in one tile or over consecutive tiles) with no
+// barrier in between: every cross-lane read sits behind the barrier that
+// follows the writes it needs.
 template <typename R, int N1, int N2, int TS, int G>
 __device__ __forceinline__ void inverse_ba(cx<R> (&w)[N1 / N2][N2], cx<R> (&v)[N1], cx<R>* lds, const cx<R>* twB,
                                            int c, int g, int t)
@@ -638,14 +642,12 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
             stage_tables<R, M, H, Sh::NT>(tabs, a, tid);
             round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H>(
                 v, vp, gm, a1, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc, eacc);
-            lds_barrier();  // every lane is done with the exchange buffer
             round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H>(
                 v, vp, gm, a2, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc);
         } else {
             const cx<R>* ph = row_phase<R>(a, gm);
             round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H>(v, vp, gm, a1, lds, twA, twB, twC,
                                                                               ph, c, gh, t, h, nacc);
-            lds_barrier();  // every lane is done with the exchange buffer
             round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H>(v, vp, gm, a2, lds, twA, twB, twC,
                                                                               ph, c, gh, t, h, nacc);
         }
@@ -776,7 +778,6 @@ __global__ __launch_bounds__(TS * G * N2 * H) void k_batch_sweep(const SweepArgs
         a1.flags = a.flags & ~kFlagNorm;
         round_trip_tile<S, R, N1, N2, TS, H * G, true, false, H>(v, vp, gm, a1, lds, twA, twB, twC, ph, c, gh, t,
                                                                  h, nacc);
-        lds_barrier();  // every lane is done with the exchange buffer
         SweepArgs a2 = a;
         a2.flags = a.flags & ~(kFlagNL | kFlagScale);
         round_trip_tile<S, R, N1, N2, TS, H * G, true, false, H>(v, vp, gm, a2, lds, twA, twB, twC, ph, c, gh, t,
@@ -854,7 +855,6 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
         // the next tile's loads were issued before these N1 stores: wait for
         // them here, with the stores still in flight
         wait_vmcnt<N1>();
-        lds_barrier();  // every lane is done with the exchange buffer
 #pragma unroll
         for (int j = 0; j < N1; ++j) v[j] = vn[j];
         gm = gn;
