@@ -809,12 +809,21 @@ int fft_impl(sfd_ctx* c, const void* in, void* out, int axis, int dir)
 }
 
 // One real-time Strang step that also stores the state before the
-// nonlinear phase (psi1 = K psi) in `save`.
+// nonlinear phase (psi1 = K psi) in `save`.  Where split_impl folds the step
+// (D >= 2, and Omega = 0 or D = 3) this runs the same launches, the double
+// round trip storing psi1, so that recomputed states equal the forward
+// pass's bit for bit.
 template <typename P>
 int strang_step_save(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt, void* save)
 {
     const int D = c->ndim;
     const double tau = 0.5 * dt;
+    if (D >= 2 && (Om == 0.0 || D == 3)) {
+        for (int ax = 0; ax < D - 1; ++ax) SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt));
+        SFD_TRY(round_trip<P>(c, psi, V, D - 1, tau, cD, Om, 0, kFlagNL, beta, dt, save, nullptr, true));
+        for (int ax = 0; ax < D - 1; ++ax) SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt));
+        return SFD_OK;
+    }
     for (int h = 0; h < 2; ++h)
         for (int ax = 0; ax < D; ++ax) {
             const bool nl = (h == 0 && ax == D - 1);
@@ -870,7 +879,7 @@ int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target
     for (int64_t k = 0; k < nck; ++k) {
         SFD_HIP(hipMemcpyAsync(fld(ck.p, k), psi.p, fb, hipMemcpyDeviceToDevice, c->stream));
         const int64_t m = std::min<int64_t>(every, N - k * every);
-        SFD_TRY(split_impl<P>(c, psi.p, V, cD, beta, Om, dt, m, 2, SFD_REAL_TIME, nullptr, false));
+        SFD_TRY(split_impl<P>(c, psi.p, V, cD, beta, Om, dt, m, 2, SFD_REAL_TIME, nullptr));
     }
     if (psi_final) SFD_HIP(hipMemcpyAsync(psi_final, psi.p, fb, hipMemcpyDeviceToDevice, c->stream));
 
@@ -892,23 +901,31 @@ int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target
     if (gradV) SFD_HIP(hipMemsetAsync(gradV, 0, (size_t)ns * sizeof(double), c->stream));
     SFD_HIP(hipMemsetAsync(accd + 2, 0, sizeof(double), c->stream));
 
-    // reverse sweep, segment by segment from the last checkpoint
+    // reverse sweep, segment by segment from the last checkpoint.  With
+    // Omega = 0 the axis flows commute, so the trailing K^dagger(tau) of one
+    // reversed step and the leading one of the step before it run as one
+    // K^dagger(2 tau); the sequence of operations on G does not depend on
+    // the checkpoint interval.
     const double tau = 0.5 * dt;
+    const bool merge = Om == 0.0;
+    bool started = false;
     for (int64_t k = nck - 1; k >= 0; --k) {
         const int64_t m = std::min<int64_t>(every, N - k * every);
         SFD_HIP(hipMemcpyAsync(psi.p, fld(ck.p, k), fb, hipMemcpyDeviceToDevice, c->stream));
         for (int64_t j = 0; j < m; ++j) SFD_TRY(strang_step_save<P>(c, psi.p, V, cD, beta, Om, dt, fld(seg.p, j)));
         for (int64_t j = m - 1; j >= 0; --j) {
-            SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
+            SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, (merge && started) ? 2.0 * tau : tau));
+            started = true;
             hipLaunchKernelGGL((k_nl_adjoint<S, R>), dim3((unsigned)nb), dim3(256), 0, c->stream,
                                static_cast<cx<S>*>(G.p), static_cast<const cx<S>*>(fld(seg.p, j)),
                                static_cast<const S*>(V), ns, (R)beta, (R)dt, gradV, partd);
             SFD_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(256), 0, c->stream, (const double*)partd, nb, accd + 2);
             SFD_HIP(hipGetLastError());
-            SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
+            if (!merge) SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
         }
     }
+    if (merge && started) SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
     double gb = 0.0;
     SFD_HIP(hipMemcpyAsync(&gb, accd + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SFD_HIP(hipStreamSynchronize(c->stream));
