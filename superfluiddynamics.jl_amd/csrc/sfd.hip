@@ -900,6 +900,8 @@ int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target
     SFD_HIP(hipGetLastError());
     if (gradV) SFD_HIP(hipMemsetAsync(gradV, 0, (size_t)ns * sizeof(double), c->stream));
     SFD_HIP(hipMemsetAsync(accd + 2, 0, sizeof(double), c->stream));
+    // dL/dbeta: per-workgroup sums over all reversed steps, reduced once
+    SFD_HIP(hipMemsetAsync(partd, 0, (size_t)nb * sizeof(double), c->stream));
 
     // reverse sweep, segment by segment from the last checkpoint.  With
     // Omega = 0 the axis flows commute, so the trailing K^dagger(tau) of one
@@ -920,12 +922,12 @@ int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target
                                static_cast<cx<S>*>(G.p), static_cast<const cx<S>*>(fld(seg.p, j)),
                                static_cast<const S*>(V), ns, (R)beta, (R)dt, gradV, partd);
             SFD_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(256), 0, c->stream, (const double*)partd, nb, accd + 2);
-            SFD_HIP(hipGetLastError());
             if (!merge) SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
         }
     }
     if (merge && started) SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
+    hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(256), 0, c->stream, (const double*)partd, nb, accd + 2);
+    SFD_HIP(hipGetLastError());
     double gb = 0.0;
     SFD_HIP(hipMemcpyAsync(&gb, accd + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SFD_HIP(hipStreamSynchronize(c->stream));

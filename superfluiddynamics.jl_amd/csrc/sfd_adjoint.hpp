@@ -14,7 +14,8 @@
 namespace sfd {
 
 // Reverse of the nonlinear phase at every site; dL/dV accumulates into
-// gradV (FP64), dL/dbeta into per-workgroup partials.
+// gradV (FP64), dL/dbeta into per-workgroup partials that persist over the
+// reverse sweep (each workgroup adds to its own slot).
 template <typename S, typename R>
 __global__ void k_nl_adjoint(cx<S>* __restrict__ G, const cx<S>* __restrict__ psi1, const S* __restrict__ V,
                              long long n, R beta, R dt, double* __restrict__ gradV, double* __restrict__ partial)
@@ -36,7 +37,7 @@ __global__ void k_nl_adjoint(cx<S>* __restrict__ G, const cx<S>* __restrict__ ps
         acc += (double)q * (double)dt * (double)a2;
     }
     double acc1[1] = {acc};
-    block_partials<R, 256, 1>(acc1, partial, gridDim.x, blockIdx.x);
+    block_partials<R, 256, 1, true>(acc1, partial, gridDim.x, blockIdx.x);
 }
 
 // Partial sums of <a|b> = sum conj(a) b (re and im, FP64).

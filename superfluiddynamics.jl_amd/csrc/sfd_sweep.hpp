@@ -136,7 +136,9 @@ __device__ __forceinline__ void nl_phase(cx<R>& v, R vpot, R beta, R dt, bool im
     }
 }
 
-template <typename R, int NT, int K>
+// Workgroup sums of acc[K] into partial[k * nblocks + block]; ACC adds them
+// to what the slot holds (a slot per workgroup: no atomics).
+template <typename R, int NT, int K, bool ACC = false>
 __device__ __forceinline__ void block_partials(double (&acc)[K], double* partial, long long nblocks,
                                                long long block)
 {
@@ -152,7 +154,8 @@ __device__ __forceinline__ void block_partials(double (&acc)[K], double* partial
     if (threadIdx.x < K) {
         double s = 0.0;
         for (int w = 0; w < (NT + 63) / 64; ++w) s += red[threadIdx.x][w];
-        partial[threadIdx.x * nblocks + block] = s;
+        if constexpr (ACC) partial[threadIdx.x * nblocks + block] += s;
+        else partial[threadIdx.x * nblocks + block] = s;
     }
 }
 
