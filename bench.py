@@ -6,7 +6,7 @@ Workload: periodic box [0, 2π)³, 256³ sites, V = 0, coeffΔ = -0.5,
 field on 1 <= |k| <= 8 (modal variance ∝ |k|^-2, rms 1).  A step is one full
 Strang step (x, y, z round trips over Δt/2, the nonlinear phase over Δt,
 x, y, z again) of the whole field, which stays resident in HBM; with Ω = 0
-it runs as x, y, [z N z], y, x.  FP32 is the headline ``value``; FP32
+it runs as x, y, [z N z], x, y.  FP32 is the headline ``value``; FP32
 storage with FP64 arithmetic and FP64 are measured side by side.
 
 Prints one JSON line.  ``python bench.py --gpus N --steps K --warmup W``.
