@@ -51,8 +51,6 @@ int fail(int code, const char* fmt, ...)
         if (r_ != SFD_OK) return r_; \
     } while (0)
 
-constexpr int lg2(int n) { return n <= 1 ? 0 : 1 + lg2(n / 2); }
-
 // ----------------------------------------------------------- small kernels
 template <typename S>
 __global__ void k_norm2(const cx<S>* __restrict__ psi, long long n, double* partial)

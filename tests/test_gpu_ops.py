@@ -1,5 +1,4 @@
 """GPU checks of the directional FFTs, reductions and full-size invariants."""
-import math
 
 import numpy as np
 import pytest

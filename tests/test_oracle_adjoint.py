@@ -4,7 +4,7 @@ the forward loss, a directional derivative in V, and the dot-product
 import numpy as np
 import pytest
 
-from helpers import mesh, random_potential, smooth_state
+from helpers import random_potential, smooth_state
 
 
 def _setup(shape=(16, 16), seed=0, Om=0.0):
