@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -234,6 +235,7 @@ struct sfd_ctx {
     long long ehist_cap = 0;
     double* scal = nullptr;     // [0] pending scale, [1..31] sums
     std::vector<PhaseEntry> phases;
+    bool pin_phases = false;    // no eviction: two streams read the cache
 
     size_t csize() const { return prec == SFD_F64 ? 16 : 8; }
     double dvol() const
@@ -345,7 +347,7 @@ int build_phase(sfd_ctx* c, int axis, double tau, double cD, double Om, int imag
     SFD_HIP(hipMalloc(&dev, h.size() * sizeof(cx<R>)));
     int rc = upload_sync(dev, h.data(), h.size() * sizeof(cx<R>));
     if (rc != SFD_OK) { (void)hipFree(dev); return rc; }
-    if (c->phases.size() >= 32) {  // bounded cache: drop the oldest entry
+    if (c->phases.size() >= 32 && !c->pin_phases) {  // bounded cache: drop the oldest entry
         (void)hipStreamSynchronize(c->stream);
         (void)hipFree(c->phases.front().dev);
         c->phases.erase(c->phases.begin());
@@ -846,10 +848,45 @@ struct DevBuf {
     }
 };
 
+// A second stream (default priority) beside the context's, with the events
+// that order them.  On destruction it gives the context back its stream and
+// phase-cache eviction, and waits for both streams, so that buffers freed
+// after it are idle.
+struct SideStream {
+    sfd_ctx* c = nullptr;
+    hipStream_t main = nullptr, side = nullptr;
+    hipEvent_t start = nullptr, ready[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr};
+    int init(sfd_ctx* ctx)
+    {
+        c = ctx;
+        main = ctx->stream;
+        SFD_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        SFD_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+        for (int b = 0; b < 2; ++b) {
+            SFD_HIP(hipEventCreateWithFlags(&ready[b], hipEventDisableTiming));
+            SFD_HIP(hipEventCreateWithFlags(&freed[b], hipEventDisableTiming));
+        }
+        return SFD_OK;
+    }
+    ~SideStream()
+    {
+        if (!c) return;
+        c->stream = main;
+        c->pin_phases = false;
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+        }
+        (void)hipStreamSynchronize(main);
+        for (hipEvent_t e : {start, ready[0], ready[1], freed[0], freed[1]})
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
 // Gradient of L = 1 - |<target|psi_N>|^2 through nsteps real-time Strang
 // steps by the reverse sweep, recomputing forward states from checkpoints
 // taken every `every` steps (sqrt(nsteps) when every <= 0): memory is
-// (nsteps/every + every) fields, forward work twice the plain run.
+// (nsteps/every + 2 every) fields, forward work twice the plain run.
 template <typename P>
 int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target, double cD, double beta, double Om,
                  double dt, int64_t N, int64_t every, double* gradV, double* grad_beta, double* loss, void* psi_final)
@@ -865,7 +902,7 @@ int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target
     SFD_HIP(hipMalloc(&psi.p, fb));
     SFD_HIP(hipMalloc(&G.p, fb));
     if (nck > 0) SFD_HIP(hipMalloc(&ck.p, fb * (size_t)nck));
-    SFD_HIP(hipMalloc(&seg.p, fb * (size_t)(every > 0 ? every : 1)));
+    SFD_HIP(hipMalloc(&seg.p, 2 * fb * (size_t)(every > 0 ? every : 1)));
     SFD_HIP(hipMalloc(&part.p, 2 * (size_t)nb * sizeof(double)));
     SFD_HIP(hipMalloc(&acc.p, 4 * sizeof(double)));
     auto fld = [&](void* base, int64_t k) { return static_cast<void*>(static_cast<char*>(base) + fb * (size_t)k); };
@@ -908,21 +945,53 @@ int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target
     // the checkpoint interval.
     const double tau = 0.5 * dt;
     const bool merge = Om == 0.0;
+    // Segment i (checkpoint k = nck - 1 - i) is recomputed on a side stream
+    // while segment i - 1 is reversed on the context's stream; the saved
+    // states alternate between two halves of `seg`.  Every phase table both
+    // read is built first, and the cache evicts nothing while they run.
+    for (int ax = 0; ax < c->ndim; ++ax) {
+        void* ph = nullptr;
+        int rows = 0;
+        SFD_TRY(build_phase<P>(c, ax, tau, cD, Om, 0, &ph, &rows));
+        SFD_TRY(build_phase<P>(c, ax, -tau, cD, Om, 0, &ph, &rows));
+        if (merge) SFD_TRY(build_phase<P>(c, ax, -2.0 * tau, cD, Om, 0, &ph, &rows));
+    }
+    SideStream ss;
+    SFD_TRY(ss.init(c));
+    auto seg_len = [&](int64_t i) { return std::min<int64_t>(every, N - (nck - 1 - i) * every); };
+    auto saved = [&](int64_t i, int64_t j) { return fld(seg.p, (i % 2) * every + j); };
+    auto recompute = [&](int64_t i) -> int {
+        if (i >= 2) SFD_HIP(hipStreamWaitEvent(ss.side, ss.freed[i % 2], 0));  // segment i - 2 reversed
+        c->stream = ss.side;
+        SFD_HIP(hipMemcpyAsync(psi.p, fld(ck.p, nck - 1 - i), fb, hipMemcpyDeviceToDevice, ss.side));
+        for (int64_t j = 0; j < seg_len(i); ++j)
+            SFD_TRY(strang_step_save<P>(c, psi.p, V, cD, beta, Om, dt, saved(i, j)));
+        c->stream = ss.main;
+        SFD_HIP(hipEventRecord(ss.ready[i % 2], ss.side));
+        return SFD_OK;
+    };
+    if (nck > 0) {
+        SFD_HIP(hipEventRecord(ss.start, ss.main));  // checkpoints written, psi free
+        SFD_HIP(hipStreamWaitEvent(ss.side, ss.start, 0));
+        c->pin_phases = true;
+        SFD_TRY(recompute(0));
+    }
     bool started = false;
-    for (int64_t k = nck - 1; k >= 0; --k) {
-        const int64_t m = std::min<int64_t>(every, N - k * every);
-        SFD_HIP(hipMemcpyAsync(psi.p, fld(ck.p, k), fb, hipMemcpyDeviceToDevice, c->stream));
-        for (int64_t j = 0; j < m; ++j) SFD_TRY(strang_step_save<P>(c, psi.p, V, cD, beta, Om, dt, fld(seg.p, j)));
-        for (int64_t j = m - 1; j >= 0; --j) {
+    for (int64_t i = 0; i < nck; ++i) {
+        if (i + 1 < nck) SFD_TRY(recompute(i + 1));
+        SFD_HIP(hipStreamWaitEvent(ss.main, ss.ready[i % 2], 0));
+        for (int64_t j = seg_len(i) - 1; j >= 0; --j) {
             SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, (merge && started) ? 2.0 * tau : tau));
             started = true;
             hipLaunchKernelGGL((k_nl_adjoint<S, R>), dim3((unsigned)nb), dim3(256), 0, c->stream,
-                               static_cast<cx<S>*>(G.p), static_cast<const cx<S>*>(fld(seg.p, j)),
+                               static_cast<cx<S>*>(G.p), static_cast<const cx<S>*>(saved(i, j)),
                                static_cast<const S*>(V), ns, (R)beta, (R)dt, gradV, partd);
             SFD_HIP(hipGetLastError());
             if (!merge) SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
         }
+        SFD_HIP(hipEventRecord(ss.freed[i % 2], ss.main));
     }
+    c->pin_phases = false;
     if (merge && started) SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
     hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(256), 0, c->stream, (const double*)partd, nb, accd + 2);
     SFD_HIP(hipGetLastError());
