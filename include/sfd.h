@@ -178,9 +178,10 @@ int sfd_batch_autotune(sfd_batch* b, float* times_ms);
  * dL/dbeta on the host and writes dL/dV(x) (FP64, nsites values) to the
  * device buffer gradV when non-NULL; psi_final (optional) receives psi_N.
  * The reverse sweep recomputes forward states from checkpoints taken every
- * checkpoint_every steps (<= 0: ceil(sqrt(nsteps))), on a second stream of
- * its own; device memory: nsteps/checkpoint_every + 2*checkpoint_every + 2
- * fields.  Synchronises. */
+ * checkpoint_every steps (<= 0: ceil(sqrt(nsteps))), on S side streams of
+ * its own (S = 2 for fields of at most 16 MiB, else 1); device memory:
+ * nsteps/checkpoint_every + (S + 1)*checkpoint_every + S + 2 fields.
+ * Synchronises. */
 int sfd_adjoint(sfd_ctx* ctx, const void* psi0, const void* V, const void* target, double coeffD,
                 double beta, double Omega, double dt, int64_t nsteps, int64_t checkpoint_every,
                 double* gradV, double* grad_beta, double* loss, void* psi_final);

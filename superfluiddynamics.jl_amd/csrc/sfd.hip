@@ -848,45 +848,52 @@ struct DevBuf {
     }
 };
 
-// A second stream (default priority) beside the context's, with the events
-// that order them.  On destruction it gives the context back its stream and
-// phase-cache eviction, and waits for both streams, so that buffers freed
-// after it are idle.
-struct SideStream {
+// Side streams (default priority) beside the context's, with the events
+// that order them: ready[b] / freed[b] per saved-state buffer b.  On
+// destruction it gives the context back its stream and phase-cache
+// eviction, and waits for every stream, so that buffers freed after it are
+// idle.
+struct SideStreams {
+    static constexpr int kMax = 2;
     sfd_ctx* c = nullptr;
-    hipStream_t main = nullptr, side = nullptr;
-    hipEvent_t start = nullptr, ready[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr};
-    int init(sfd_ctx* ctx)
+    int n = 0;
+    hipStream_t main = nullptr, side[kMax] = {nullptr, nullptr};
+    hipEvent_t start = nullptr, ready[kMax + 1] = {}, freed[kMax + 1] = {};
+    int init(sfd_ctx* ctx, int nside)
     {
         c = ctx;
         main = ctx->stream;
-        SFD_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         SFD_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming));
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b <= nside; ++b) {
             SFD_HIP(hipEventCreateWithFlags(&ready[b], hipEventDisableTiming));
             SFD_HIP(hipEventCreateWithFlags(&freed[b], hipEventDisableTiming));
         }
+        for (; n < nside; ++n) SFD_HIP(hipStreamCreateWithFlags(&side[n], hipStreamNonBlocking));
         return SFD_OK;
     }
-    ~SideStream()
+    ~SideStreams()
     {
         if (!c) return;
         c->stream = main;
         c->pin_phases = false;
-        if (side) {
-            (void)hipStreamSynchronize(side);
-            (void)hipStreamDestroy(side);
+        for (int k = 0; k < n; ++k) {
+            (void)hipStreamSynchronize(side[k]);
+            (void)hipStreamDestroy(side[k]);
         }
         (void)hipStreamSynchronize(main);
-        for (hipEvent_t e : {start, ready[0], ready[1], freed[0], freed[1]})
-            if (e) (void)hipEventDestroy(e);
+        if (start) (void)hipEventDestroy(start);
+        for (int b = 0; b <= kMax; ++b) {
+            if (ready[b]) (void)hipEventDestroy(ready[b]);
+            if (freed[b]) (void)hipEventDestroy(freed[b]);
+        }
     }
 };
 
 // Gradient of L = 1 - |<target|psi_N>|^2 through nsteps real-time Strang
 // steps by the reverse sweep, recomputing forward states from checkpoints
 // taken every `every` steps (sqrt(nsteps) when every <= 0): memory is
-// (nsteps/every + 2 every) fields, forward work twice the plain run.
+// nsteps/every + (S + 1) every + S + 2 fields with S side streams (2 for
+// fields of at most 16 MiB, else 1), forward work twice the plain run.
 template <typename P>
 int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target, double cD, double beta, double Om,
                  double dt, int64_t N, int64_t every, double* gradV, double* grad_beta, double* loss, void* psi_final)
@@ -898,11 +905,15 @@ int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target
     if (every <= 0) every = (int64_t)std::ceil(std::sqrt((double)(N > 0 ? N : 1)));
     const int64_t nck = N > 0 ? (N + every - 1) / every : 0;
     const long long nb = grid_blocks(ns);
+    // Small fields leave most CUs idle in every sweep: two side streams then
+    // recompute two segments at once.
+    const int nside = fb <= ((size_t)16 << 20) ? 2 : 1;
+    const int nbuf = nside + 1;
     DevBuf psi, G, ck, seg, part, acc;
-    SFD_HIP(hipMalloc(&psi.p, fb));
+    SFD_HIP(hipMalloc(&psi.p, fb * (size_t)nside));
     SFD_HIP(hipMalloc(&G.p, fb));
     if (nck > 0) SFD_HIP(hipMalloc(&ck.p, fb * (size_t)nck));
-    SFD_HIP(hipMalloc(&seg.p, 2 * fb * (size_t)(every > 0 ? every : 1)));
+    SFD_HIP(hipMalloc(&seg.p, nbuf * fb * (size_t)(every > 0 ? every : 1)));
     SFD_HIP(hipMalloc(&part.p, 2 * (size_t)nb * sizeof(double)));
     SFD_HIP(hipMalloc(&acc.p, 4 * sizeof(double)));
     auto fld = [&](void* base, int64_t k) { return static_cast<void*>(static_cast<char*>(base) + fb * (size_t)k); };
@@ -945,10 +956,11 @@ int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target
     // the checkpoint interval.
     const double tau = 0.5 * dt;
     const bool merge = Om == 0.0;
-    // Segment i (checkpoint k = nck - 1 - i) is recomputed on a side stream
-    // while segment i - 1 is reversed on the context's stream; the saved
-    // states alternate between two halves of `seg`.  Every phase table both
-    // read is built first, and the cache evicts nothing while they run.
+    // Segment i (checkpoint k = nck - 1 - i) is recomputed on side stream
+    // i % nside, up to nside segments ahead of the one reversed on the
+    // context's stream; its saved states go to buffer i % nbuf of `seg`.
+    // Every phase table the streams read is built first, and the cache
+    // evicts nothing while they run.
     for (int ax = 0; ax < c->ndim; ++ax) {
         void* ph = nullptr;
         int rows = 0;
@@ -956,30 +968,32 @@ int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target
         SFD_TRY(build_phase<P>(c, ax, -tau, cD, Om, 0, &ph, &rows));
         if (merge) SFD_TRY(build_phase<P>(c, ax, -2.0 * tau, cD, Om, 0, &ph, &rows));
     }
-    SideStream ss;
-    SFD_TRY(ss.init(c));
+    SideStreams ss;
+    SFD_TRY(ss.init(c, nside));
     auto seg_len = [&](int64_t i) { return std::min<int64_t>(every, N - (nck - 1 - i) * every); };
-    auto saved = [&](int64_t i, int64_t j) { return fld(seg.p, (i % 2) * every + j); };
+    auto saved = [&](int64_t i, int64_t j) { return fld(seg.p, (i % nbuf) * every + j); };
     auto recompute = [&](int64_t i) -> int {
-        if (i >= 2) SFD_HIP(hipStreamWaitEvent(ss.side, ss.freed[i % 2], 0));  // segment i - 2 reversed
-        c->stream = ss.side;
-        SFD_HIP(hipMemcpyAsync(psi.p, fld(ck.p, nck - 1 - i), fb, hipMemcpyDeviceToDevice, ss.side));
+        hipStream_t st = ss.side[i % nside];
+        void* work = fld(psi.p, i % nside);
+        if (i >= nbuf) SFD_HIP(hipStreamWaitEvent(st, ss.freed[i % nbuf], 0));  // segment i - nbuf reversed
+        c->stream = st;
+        SFD_HIP(hipMemcpyAsync(work, fld(ck.p, nck - 1 - i), fb, hipMemcpyDeviceToDevice, st));
         for (int64_t j = 0; j < seg_len(i); ++j)
-            SFD_TRY(strang_step_save<P>(c, psi.p, V, cD, beta, Om, dt, saved(i, j)));
+            SFD_TRY(strang_step_save<P>(c, work, V, cD, beta, Om, dt, saved(i, j)));
         c->stream = ss.main;
-        SFD_HIP(hipEventRecord(ss.ready[i % 2], ss.side));
+        SFD_HIP(hipEventRecord(ss.ready[i % nbuf], st));
         return SFD_OK;
     };
+    int64_t queued = 0;  // segments whose recompute is enqueued
     if (nck > 0) {
         SFD_HIP(hipEventRecord(ss.start, ss.main));  // checkpoints written, psi free
-        SFD_HIP(hipStreamWaitEvent(ss.side, ss.start, 0));
+        for (int k = 0; k < nside; ++k) SFD_HIP(hipStreamWaitEvent(ss.side[k], ss.start, 0));
         c->pin_phases = true;
-        SFD_TRY(recompute(0));
     }
     bool started = false;
     for (int64_t i = 0; i < nck; ++i) {
-        if (i + 1 < nck) SFD_TRY(recompute(i + 1));
-        SFD_HIP(hipStreamWaitEvent(ss.main, ss.ready[i % 2], 0));
+        for (; queued < nck && queued <= i + nside; ++queued) SFD_TRY(recompute(queued));
+        SFD_HIP(hipStreamWaitEvent(ss.main, ss.ready[i % nbuf], 0));
         for (int64_t j = seg_len(i) - 1; j >= 0; --j) {
             SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, (merge && started) ? 2.0 * tau : tau));
             started = true;
@@ -989,7 +1003,7 @@ int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target
             SFD_HIP(hipGetLastError());
             if (!merge) SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
         }
-        SFD_HIP(hipEventRecord(ss.freed[i % 2], ss.main));
+        SFD_HIP(hipEventRecord(ss.freed[i % nbuf], ss.main));
     }
     c->pin_phases = false;
     if (merge && started) SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
