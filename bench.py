@@ -44,22 +44,33 @@ def bytes_per_site_update(prec: str, ndim: int = 3, with_v: bool = False, passes
     return passes * 2 * sc + (sr if with_v else 0)
 
 
-def load_traffic(name: str):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration recorded with it); None if no summary was committed."""
-    path = os.path.join(ROOT, "profiles", name)
+def load_traffic(slot: str):
+    """HBM bytes per launch of the kernel of sweep slot `slot` from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration recorded with it); None if it holds no figure
+    for that kernel."""
+    path = os.path.join(ROOT, "profiles", "traffic_256cube_f32.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         d = json.load(fh)
-    return d.get("traffic_bytes_per_launch")
+    k = d.get("kernels", {}).get(slot)
+    return k.get("traffic_bytes_per_launch") if k else None
 
 
 PRECISIONS = {"f32": ("float32", None), "f32x": ("float32", "float64"), "f64": ("float64", None)}
 
 
-def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, kernel_launches, autotune=True):
+SWEEP_SLOTS = ("x round trip", "y round trip", "z round trip", "double round trip [d N d]", "other sweeps")
+
+
+def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, reps, autotune=True):
+    """One precision: autotune, warm up, then `reps` timed runs of `steps`
+    Split2 steps (HIP events on the context's stream, which is torch's
+    current stream); ms/step is their median.  A further run of `steps`
+    steps with the library's sweep timer gives the average duration of each
+    kernel of the step, measured inside the step sequence itself."""
+    import ctypes
     import numpy as np
     FT, compute = PRECISIONS[prec]
     n = psi0.shape[0]
@@ -72,42 +83,37 @@ def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, kernel_launches, autotune
     model = sfd.NumModelSplit2(f, p, mp["dt"], steps, steps)
     model.steps(max(warmup, 1))
     torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record()
-    model.steps(steps)
-    e1.record()
-    torch.cuda.synchronize()
-    ms_step = e0.elapsed_time(e1) / steps
+    runs = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        model.steps(steps)
+        e1.record()
+        torch.cuda.synchronize()
+        runs.append(e0.elapsed_time(e1) / steps)
+    ms_step = float(np.median(runs))
     state = f.to_numpy()
     nrm = sfd.norm(f)
     solve = solve_loop(sfd, torch, model, min(steps, 50))
-    # Average duration of one launch of each sweep kernel, on the stream the
-    # kernels run on (torch's current stream is the context's stream).
-    lib = sfd.load()
-    import ctypes
-    ctx = f.ctx()
-    per_axis = []
-    for ax in range(3):
-        rc = lib.sfd_lap_rot_axis(ctx.h, ctypes.c_void_p(f.ϕ.data_ptr()), ax, mp["coeffD"], 0.0, mp["dt"] * 0.5, 0)
-        assert rc == 0, lib.sfd_last_error()
-        reps = []
-        for _ in range(3):
-            k0 = torch.cuda.Event(enable_timing=True)
-            k1 = torch.cuda.Event(enable_timing=True)
-            torch.cuda.synchronize()
-            k0.record()
-            for _ in range(kernel_launches):
-                lib.sfd_lap_rot_axis(ctx.h, ctypes.c_void_p(f.ϕ.data_ptr()), ax, mp["coeffD"], 0.0,
-                                     mp["dt"] * 0.5, 0)
-            k1.record()
-            torch.cuda.synchronize()
-            reps.append(k0.elapsed_time(k1) / kernel_launches)
-        per_axis.append(sorted(reps)[1])
+    lib, ctx = sfd.load(), f.ctx()
+    sfd._lib.check(lib.sfd_ctx_time_sweeps(ctx.h, 1))
+    sfd._lib.check(lib.sfd_ctx_sweep_times(ctx.h, None, None, 1))
+    model.steps(steps)
+    tot = (ctypes.c_double * 5)()
+    cnt = (ctypes.c_int64 * 5)()
+    sfd._lib.check(lib.sfd_ctx_sweep_times(ctx.h, tot, cnt, 1))
+    sfd._lib.check(lib.sfd_ctx_time_sweeps(ctx.h, 0))
+    sweeps = {}
+    for k, name in enumerate(SWEEP_SLOTS):
+        if cnt[k]:
+            sweeps[name] = {"launches_per_step": cnt[k] / steps, "avg_ms": tot[k] / cnt[k],
+                            "ms_per_step": tot[k] / steps}
     del model, f
     torch.cuda.empty_cache()
-    return ms_step, per_axis, state, nrm, variants, tune, solve
+    return dict(ms=ms_step, runs_ms=runs, state=state, norm=nrm, variants=variants, tune=tune, solve=solve,
+                sweeps=sweeps)
 
 
 def solve_loop(sfd, torch, model, k):
@@ -128,9 +134,14 @@ def solve_loop(sfd, torch, model, k):
     return {"iterations": k, "separate_ms_per_iter": sep * 1e3, "fused_ms_per_iter": fused * 1e3}
 
 
-def sweep_kernel_name(v):
-    """Kernel of a variant (n1, n2, ts, g, pipelined, waves)."""
-    n1, n2, ts, g, pipe = v[:5]
+def kernel_name(variants, slot):
+    """Kernel of sweep slot `slot` (0-2: the round trip of axis x, y, z; 3: the
+    double round trip of the innermost axis) under the FP32 variants
+    (n1, n2, ts, g, pipelined, waves, halves) per axis."""
+    ax = 2 if slot == 3 else slot
+    n1, n2, ts, g, pipe = variants[ax][:5]
+    if slot == 3:
+        return f"k_sweep<float, float, {n1}, {n2}, {ts}, {g}, 7>"
     return f"{'k_sweep_pipe' if pipe else 'k_sweep'}<float, float, {n1}, {n2}, {ts}, {g}>"
 
 
@@ -260,7 +271,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel-launches", type=int, default=40)
+    ap.add_argument("--reps", type=int, default=5, help="timed runs of --steps steps; ms/step is their median")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the 1000-step FP32/F32X-vs-FP64 accuracy run at the stable dt")
@@ -297,17 +308,19 @@ def main(argv=None):
     res = {}
     states = {}
     for prec in ("f32", "f32x", "f64"):
-        ms, per_axis, st, nrm, variants, tune, solve = gpu_leg(sfd, torch, psi0, mp, prec, args.steps,
-                                                                args.warmup, args.kernel_launches,
-                                                                not args.no_autotune)
-        res[prec] = dict(ms=ms, per_axis=per_axis, norm=nrm, variants=variants, tune=tune, solve=solve)
+        r = gpu_leg(sfd, torch, psi0, mp, prec, args.steps, args.warmup, args.reps, not args.no_autotune)
+        states[prec] = r.pop("state")
+        res[prec] = r
+        solve = r["solve"]
         print(f"[bench] {prec}: solve! iteration {solve['separate_ms_per_iter']:.4f} ms with a separate energy, "
               f"{solve['fused_ms_per_iter']:.4f} ms fused", file=sys.stderr, flush=True)
-        print(f"[bench] {prec}: variants (n1,n2,ts,g) per axis {variants}; autotune ms {tune}",
-              file=sys.stderr, flush=True)
-        states[prec] = st
-        print(f"[bench] {prec}: {ms:.4f} ms/step, sweeps x/y/z {['%.4f' % v for v in per_axis]} ms",
-              file=sys.stderr, flush=True)
+        print(f"[bench] {prec}: variants (n1,n2,ts,g,pipe,waves,halves) per axis {r['variants']}; "
+              f"autotune ms {r['tune']}", file=sys.stderr, flush=True)
+        print(f"[bench] {prec}: {r['ms']:.4f} ms/step (median of {args.reps} runs of {args.steps} steps: "
+              f"{['%.4f' % v for v in r['runs_ms']]}); kernels in the step: "
+              + ", ".join(f"{k} {v['avg_ms'] * 1e3:.1f} us x {v['launches_per_step']:.0f}"
+                          for k, v in r["sweeps"].items()), file=sys.stderr, flush=True)
+
     def rel(a, b):
         return float(np.sqrt(np.sum(np.abs(a.astype(np.complex128) - b) ** 2) / np.sum(np.abs(b) ** 2)))
     err32 = rel(states["f32"], states["f64"])
@@ -322,18 +335,20 @@ def main(argv=None):
         p_ = "f64" if prec == "f64" else "f32"
         b = bytes_per_site_update(p_, passes=5)
         b6 = bytes_per_site_update(p_)
-        out[prec] = dict(site_updates_per_s=sups, ms_per_step=r["ms"], bytes_per_site_update=b,
-                         step_hbm_frac=sups * b / HBM_PEAK, step_hbm_frac_6_pass_bytes=sups * b6 / HBM_PEAK,
-                         sweep_ms=dict(x=r["per_axis"][0], y=r["per_axis"][1], z=r["per_axis"][2],
-                                       z_fold_from_step=r["ms"] - 2 * (r["per_axis"][0] + r["per_axis"][1])),
+        out[prec] = dict(site_updates_per_s=sups, ms_per_step=r["ms"], runs_ms=r["runs_ms"],
+                         bytes_per_site_update=b, step_hbm_frac=sups * b / HBM_PEAK,
+                         step_hbm_frac_6_pass_bytes=sups * b6 / HBM_PEAK, kernels=r["sweeps"],
                          variants=r["variants"])
 
-    # Dominant kernel: the strided-axis sweep (y and z are 4 of the 6 launches).
-    r32 = res["f32"]
-    dom_ms = 0.5 * (r32["per_axis"][1] + r32["per_axis"][2])
-    launch_bytes = 2 * 8 * sites  # one read + one write of the complex FP32 field
-    achieved = launch_bytes / (dom_ms * 1e-3)
-    traffic = load_traffic("traffic_sweep_strided_f32_256.json")
+    # Dominant kernel: the sweep kernel with the largest share of the FP32
+    # step's time, timed inside the step (one read and one write of the
+    # complex FP32 field per launch; V = 0, so no potential bytes).
+    k32 = res["f32"]["sweeps"]
+    dom = max((k for k in k32 if k != "other sweeps"), key=lambda k: k32[k]["ms_per_step"])
+    launch_bytes = 2 * 8 * sites
+    achieved = launch_bytes / (k32[dom]["avg_ms"] * 1e-3)
+    traffic = load_traffic(dom)
+    dom_kernel = kernel_name(res["f32"]["variants"], SWEEP_SLOTS.index(dom))
 
     # Accuracy at the largest stable step: at dt = 5e-4 the unfiltered split is
     # past its stability limit (D |coeffD| kmax^2 dt = 12.3 > pi) and any two
@@ -374,10 +389,11 @@ def main(argv=None):
                                f"β=1/ξ² (ξ=2Δ), Δt=5e-4, random-phase ψ0 seed 0",
                    "grid": [n, n, n], "parallelism": "single GPU, whole domain resident"},
         "fp64": {"value": out["f64"]["site_updates_per_s"], "ms_per_step": out["f64"]["ms_per_step"],
-                 "hbm_frac": out["f64"]["step_hbm_frac"], "sweep_ms": out["f64"]["sweep_ms"],
-                 "variants": out["f64"]["variants"]},
+                 "runs_ms": out["f64"]["runs_ms"], "hbm_frac": out["f64"]["step_hbm_frac"],
+                 "kernels": out["f64"]["kernels"], "variants": out["f64"]["variants"]},
         "f32x": {"value": out["f32x"]["site_updates_per_s"], "ms_per_step": out["f32x"]["ms_per_step"],
-                 "hbm_frac": out["f32x"]["step_hbm_frac"], "sweep_ms": out["f32x"]["sweep_ms"],
+                 "runs_ms": out["f32x"]["runs_ms"], "hbm_frac": out["f32x"]["step_hbm_frac"],
+                 "kernels": out["f32x"]["kernels"],
                  "variants": out["f32x"]["variants"],
                  "note": "complex FP32 field in HBM, FP64 arithmetic and tables in registers/LDS"},
         "f32_over_f64_speedup": out["f64"]["ms_per_step"] / out["f32"]["ms_per_step"],
@@ -386,7 +402,8 @@ def main(argv=None):
         "f32_step_hbm_frac_note": "the step moves 5 passes x 16 B = 80 B per site-update (x, y, [z N z], x, y); "
                                   "priced at the 6-pass 96 B of the reference's structure it would read "
                                   f"{out['f32']['step_hbm_frac_6_pass_bytes']:.3f}",
-        "f32_sweep_ms": out["f32"]["sweep_ms"],
+        "f32_kernels": out["f32"]["kernels"],
+        "f32_runs_ms": out["f32"]["runs_ms"],
         "f32_variants": out["f32"]["variants"],
         "f32_vs_f64_rel_l2": err32,
         "f32x_vs_f64_rel_l2": err32x,
@@ -402,8 +419,10 @@ def main(argv=None):
                        **{p_: res[p_]["solve"] for p_ in ("f32", "f32x", "f64")}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": sweep_kernel_name(res["f32"]["variants"][1]) + " strided (y/z round trip), FP32, 256³",
-                     "bytes_per_launch": launch_bytes},
+                     "kernel": f"{dom_kernel}: {dom}, FP32, {n}³ (the step's largest share of time, "
+                               f"{k32[dom]['ms_per_step'] * 1e3:.1f} us of {out['f32']['ms_per_step'] * 1e3:.1f} "
+                               f"us per step)",
+                     "avg_launch_ms": k32[dom]["avg_ms"], "bytes_per_launch": launch_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

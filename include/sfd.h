@@ -140,6 +140,19 @@ int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* sha
  * floating-point rounding. */
 int sfd_ctx_set_variant(sfd_ctx* ctx, int axis, int index);
 
+/* Sweep timer (no counterpart in the reference, whose benchmark scripts time
+ * whole steps, benchmarks/scalability_gp.jl:36-51): with enable != 0 every
+ * sweep launch of the context is bracketed by two HIP events on its stream
+ * and its duration summed per slot: 0, 1, 2 = one-axis round trips of x, y,
+ * z; 3 = the double round trip [d N d] of a Strang step; 4 = forward,
+ * inverse and energy sweeps.  The events cost a few microseconds between
+ * launches, so time whole steps with the timer off. */
+int sfd_ctx_time_sweeps(sfd_ctx* ctx, int enable);
+/* ms_sum[5] and count[5] (either may be NULL) receive the summed durations
+ * and launch counts per slot since the last reset; reset != 0 clears them.
+ * Synchronises on the recorded events. */
+int sfd_ctx_sweep_times(sfd_ctx* ctx, double* ms_sum, int64_t* count, int reset);
+
 /* Batched imaginary-time ground states (no single counterpart in the
  * reference, which solves one field at a time with NumModelBackwardEuler,
  * backward-euler.jl:72-90): ninst independent 2D problems of possibly

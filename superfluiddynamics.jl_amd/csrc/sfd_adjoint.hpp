@@ -70,7 +70,7 @@ __global__ void k_terminal(cx<S>* __restrict__ G, const cx<S>* __restrict__ targ
 }
 
 // acc[0] += sum of nb partials (fixed order).
-__global__ void k_accumulate(const double* __restrict__ partial, long long nb, double* acc)
+static __global__ void k_accumulate(const double* __restrict__ partial, long long nb, double* acc)
 {
     __shared__ double red[256];
     double s = 0.0;
@@ -85,7 +85,7 @@ __global__ void k_accumulate(const double* __restrict__ partial, long long nb, d
 }
 
 // out[j] partials of sum_x f[x] basis[j*n + x]: blockIdx.y = j.
-__global__ void k_project(const double* __restrict__ f, const double* __restrict__ basis, long long n,
+static __global__ void k_project(const double* __restrict__ f, const double* __restrict__ basis, long long n,
                           double* __restrict__ partial)
 {
     const double* b = basis + (long long)blockIdx.y * n;

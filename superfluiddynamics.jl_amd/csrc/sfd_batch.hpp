@@ -18,7 +18,7 @@ struct BatchInstDev {
 // Per instance: pending renormalisation 1/||psi|| from the partial sums of
 // the step's last sweep; counts the step.  One workgroup per instance,
 // fixed summation order.
-__global__ void k_batch_finalize(const BatchInstDev* __restrict__ inst, double* __restrict__ scale,
+static __global__ void k_batch_finalize(const BatchInstDev* __restrict__ inst, double* __restrict__ scale,
                                  const int* __restrict__ done, long long* __restrict__ steps)
 {
     const int i = blockIdx.x;
@@ -80,7 +80,7 @@ __global__ void k_batch_diff(const BatchInstDev* __restrict__ inst, const double
 }
 
 // ||Delta psi|| = sqrt(sum * dv); an instance below tol is done.
-__global__ void k_batch_diff_finalize(const double* __restrict__ dpart, int nb,
+static __global__ void k_batch_diff_finalize(const double* __restrict__ dpart, int nb,
                                       const BatchInstDev* __restrict__ inst, double* __restrict__ dpsi,
                                       int* __restrict__ done, double tol)
 {
@@ -117,7 +117,7 @@ __global__ void k_batch_apply_scale(const BatchInstDev* __restrict__ inst, const
 
 // scale[i] = 1, done[i] = 0, steps[i] = 0 (start of a solve); kernel, not a
 // memset, so that the sequence can be captured in a graph.
-__global__ void k_batch_reset(double* scale, int* done, long long* steps, double* dpsi, int n, int full)
+static __global__ void k_batch_reset(double* scale, int* done, long long* steps, double* dpsi, int n, int full)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;

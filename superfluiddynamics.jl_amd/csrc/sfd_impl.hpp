@@ -1,0 +1,1149 @@
+// sfd_impl.hpp -- the precision-dependent routines of libsfdhip.so, templated
+// on the precision traits P (PF32, PF64, PF32X of sfd_variants.hpp):
+// phase tables, sweep launches, split steps, energies, directional FFTs, the
+// adjoint, the autotuners and the batched imaginary-time solver.  Each of
+// sfd_f32.hip, sfd_f64.hip and sfd_f32x.hip instantiates them for one
+// precision through make_ops<P>().
+#pragma once
+#include "sfd_runtime.hpp"
+
+namespace sfd {
+
+// Twiddles in the layout k_sweep reads for a line of n = h*m points split
+// as m = n1*n2 (h = 1: m = n), with w_m = exp(-2 pi i / m):
+//   [0, m):    twA[s][t] = w_m^(s t), s < n1, t < n2
+//   [m, 2m):   twB[q][s] = w_m^(q s), q < n2, s < n1
+//   [2m, 3m):  h = 2 only, the radix-2 stage's twC[k] = w_n^k, k < m
+// FP64 entries are (cos, sin) of the angle a; FP32 entries are the lifting
+// coefficients (tan(b/2), sin b) of b = a/2 with a taken in (-pi, pi].
+template <typename R>
+void fill_twiddles(std::vector<cx<R>>& w, int n, int n1, int n2, int h)
+{
+    const int m = n1 * n2;
+    w.resize((size_t)(h == 2 ? 3 : 2) * m);
+    const long double pi = 3.14159265358979323846264338327950288L;
+    // entry i = exp(-2 pi i e / d)
+    auto put = [&](size_t i, long long e, long long d) {
+        e %= d;
+        const long double a = -2.0L * pi * (long double)e / (long double)d;
+        if constexpr (sizeof(R) == 4) {
+            // lifting coefficients of the half angle b in (-pi/2, pi/2]
+            const long double b = 0.5L * (2 * e > d ? a + 2.0L * pi : a);
+            w[i].re = (R)tanl(0.5L * b);
+            w[i].im = (R)sinl(b);
+        } else {
+            w[i].re = (R)cosl(a);
+            w[i].im = (R)sinl(a);
+        }
+    };
+    for (int s = 0; s < n1; ++s)
+        for (int t = 0; t < n2; ++t) put((size_t)s * n2 + t, (long long)t * s, m);
+    for (int q = 0; q < n2; ++q)
+        for (int s = 0; s < n1; ++s) put((size_t)m + (size_t)q * n1 + s, (long long)s * q, m);
+    if (h == 2)
+        for (int k = 0; k < m; ++k) put((size_t)2 * m + k, k, n);
+}
+
+// One imaginary-time multiplier exp(arg)/n: itself in FP64 tables, and
+// expm1(arg)/n in FP32 tables, whose sweeps form X/n + X expm1(arg)/n.
+template <typename R>
+cx<R> decay_entry(double arg, double inv_n)
+{
+    if constexpr (sizeof(R) == 4) return {(R)(std::expm1(arg) * inv_n), (R)0};
+    return {(R)(std::exp(arg) * inv_n), (R)0};
+}
+
+// Phase multiplier of one axis (splitting.jl:190,195,200), divided by n:
+//   x: exp(s (cD xi^2 - Om y xi) tau)   rows indexed by y when Om != 0
+//   y: exp(s (cD xi^2 + Om x xi) tau)   rows indexed by x when Om != 0
+//   z: exp(s  cD xi^2 tau)
+// s = i (real time) or 1 (imaginary time).  Formed on the device in FP64
+// with the reference's operation order from the context's own FP64
+// wavenumber and coordinate tables, rounded once to the table precision R:
+//   FP64 tables, real time:  (cos arg, sin arg) / n
+//   FP32 tables, real time:  lifting coefficients (tan(b/2), sin b) of the
+//                            half phase b = phi/2, phi = arg reduced to
+//                            (-pi, pi]; the sweep applies them twice
+//   imaginary time:          exp(arg)/n (FP64), expm1(arg)/n (FP32: the
+//                            sweep forms X/n + X expm1(arg)/n)
+template <typename R>
+__global__ void k_build_phase(cx<R>* __restrict__ out, const double* __restrict__ xi, const double* __restrict__ co,
+                              int n, int rows, int axis, double tau, double cD, double Om, int imag)
+{
+    const long long tot = (long long)rows * n;
+    const double inv_n = 1.0 / (double)n;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(i / n), k = (int)(i - (long long)r * n);
+        const double x = xi[k];
+        const double c = rows > 1 ? co[r] : 0.0;
+        double arg;
+        if (axis == 0) arg = (cD * (x * x) - (Om * c) * x) * tau;
+        else if (axis == 1) arg = (cD * (x * x) + (Om * c) * x) * tau;
+        else arg = (cD * (x * x)) * tau;
+        cx<R> e;
+        if (imag) {
+            if constexpr (sizeof(R) == 4) e = {(R)(expm1(arg) * inv_n), (R)0};
+            else e = {(R)(exp(arg) * inv_n), (R)0};
+        } else if constexpr (sizeof(R) == 4) {
+            const double b = 0.5 * remainder(arg, 6.283185307179586476925);
+            e = {(R)tan(0.5 * b), (R)sin(b)};
+        } else {
+            double sn, cs;
+            sincos(arg, &sn, &cs);
+            e = {(R)(cs * inv_n), (R)(sn * inv_n)};
+        }
+        out[i] = e;
+    }
+}
+
+// Enqueue the build of a phase table into `dev` (rows * n[axis] entries of
+// cx<R>) on `stream`.
+template <typename P>
+int launch_build_phase(const sfd_ctx* c, hipStream_t stream, void* dev, int axis, double tau, double cD, double Om,
+                       int imag, int rows)
+{
+    using R = typename P::R;
+    const int n = c->n[axis];
+    const long long tot = (long long)rows * n;
+    const long long nb = std::min<long long>((tot + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_build_phase<R>, dim3((unsigned)nb), dim3(256), 0, stream, static_cast<cx<R>*>(dev),
+                       (const double*)c->xi[axis], (const double*)c->coord[axis == 0 ? 1 : 0], n, rows, axis, tau,
+                       cD, Om, imag);
+    SFD_HIP(hipGetLastError());
+    return SFD_OK;
+}
+
+// Phase table of (axis, tau, coeffD, Omega, mode) from the context's cache
+// (least recently used first out, at most 32 entries unless pinned), built
+// on the context's stream on a miss.
+template <typename P>
+int build_phase(sfd_ctx* c, int axis, double tau, double cD, double Om, int imag, void** out, int* rows_out)
+{
+    using R = typename P::R;
+    // a hit moves the entry to the back
+    for (size_t i = 0; i < c->phases.size(); ++i) {
+        const PhaseEntry p = c->phases[i];
+        if (p.axis == axis && p.tau == tau && p.cD == cD && p.Om == Om && p.imag == imag) {
+            c->phases.erase(c->phases.begin() + i);
+            c->phases.push_back(p);
+            *out = p.dev;
+            *rows_out = p.rows;
+            return SFD_OK;
+        }
+    }
+    const int rows = phase_rows(c, axis, Om);
+    const size_t bytes = (size_t)rows * c->n[axis] * sizeof(cx<R>);
+    if (c->phases.size() >= 32 && !c->pin_phases) {  // a sweep may still read an evicted table
+        SFD_HIP(hipStreamSynchronize(c->stream));
+        while (c->phases.size() >= 32) {
+            (void)hipFree(c->phases.front().dev);
+            c->phases.erase(c->phases.begin());
+        }
+    }
+    void* dev = nullptr;
+    SFD_HIP(hipMalloc(&dev, bytes));
+    int rc = launch_build_phase<P>(c, c->stream, dev, axis, tau, cD, Om, imag, rows);
+    if (rc != SFD_OK) {
+        (void)hipFree(dev);
+        return rc;
+    }
+    c->phases.push_back({axis, tau, cD, Om, imag, dev, rows});
+    *out = dev;
+    *rows_out = rows;
+    return SFD_OK;
+}
+
+template <typename P, int MODE>
+int launch_sweep(sfd_ctx* c, int axis, SweepArgs a)
+{
+    // The pipelined kernel of a pipelined variant takes round trips without a
+    // nonlinear epilogue, pending scale or energy sums and with one phase
+    // row; other sweeps run k_sweep.
+    const bool persistent = c->vi[axis].pipe && (MODE == kRoundTrip || MODE == kRoundTripImag) &&
+                            !(a.flags & (kFlagNL | kFlagScale)) && a.phase_row_stride == 0;
+    Launch L = sweep_geometry(c, axis, a, persistent);
+    a.tw = c->tw[axis];
+    if (c->time_sweeps) SFD_TRY(timer_begin(c, rt_twice(MODE) ? 3 : (is_round_trip(MODE) ? axis : 4)));
+    if (!launch_variant<P, MODE>(c->n[axis], axis == 0, c->var[axis], L.grid, L.block, c->stream, a, persistent))
+        return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", c->var[axis], c->n[axis]);
+    SFD_HIP(hipGetLastError());
+    if (c->time_sweeps) SFD_TRY(timer_end(c));
+    c->last_nblocks = L.nblocks;
+    return SFD_OK;
+}
+
+// Select launch variant `idx` for `axis` and (re)build the twiddle table of
+// its line split.
+template <typename P>
+int set_variant(sfd_ctx* c, int axis, int idx)
+{
+    using R = typename P::R;
+    const int n = c->n[axis];
+    VariantInfo vi;
+    if (!variant_info<P>(n, axis == 0, idx, &vi))
+        return fail(SFD_ERR_ARG, "axis %d: variant %d does not exist (%d variants)", axis, idx,
+                    n_variants<P>(n, axis == 0));
+    c->pipe_occ[axis] = vi.pipe ? variant_occupancy<P>(n, axis == 0, idx) : 0;
+    if (c->tw[axis] && c->vi[axis].n1 == vi.n1 && c->vi[axis].n2 == vi.n2 && c->vi[axis].h == vi.h) {
+        c->var[axis] = idx;
+        c->vi[axis] = vi;
+        return SFD_OK;
+    }
+    std::vector<cx<R>> w;
+    fill_twiddles<R>(w, n, vi.n1, vi.n2, vi.h);
+    if (c->tw[axis]) {
+        SFD_HIP(hipStreamSynchronize(c->stream));
+        SFD_HIP(hipFree(c->tw[axis]));
+        c->tw[axis] = nullptr;
+    }
+    SFD_HIP(hipMalloc(&c->tw[axis], w.size() * sizeof(cx<R>)));
+    SFD_TRY(upload_sync(c->tw[axis], w.data(), w.size() * sizeof(cx<R>)));
+    c->var[axis] = idx;
+    c->vi[axis] = vi;
+    return SFD_OK;
+}
+
+// One linear round trip along `axis` over tau, with optional fused flags.
+// With `epartial` (room for 4 * sweep_nblocks doubles), the sweep also
+// writes the energy sums of the state it loads there, 4 per workgroup
+// (kRoundTripEnergy; kFlagPointwise in `flags` adds the pointwise sums).
+// With `twice`, the sweep is K(tau) N K(tau) on this axis (kRoundTripTwice):
+// the nonlinear phase of `flags` between two round trips.  With `ws`, the
+// phase table, scale and partials come from it (its tables are for tau).
+template <typename P>
+int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, double cD, double Om, int imag,
+               int flags, double beta, double dt, void* save = nullptr, double* epartial = nullptr,
+               bool twice = false, const Workspace* ws = nullptr)
+{
+    using R = typename P::R;
+    SweepArgs a;
+    std::memset(&a, 0, sizeof a);
+    void* ph = nullptr;
+    int rows = 1;
+    if (ws) {
+        ph = ws->phase[axis];
+        rows = ws->rows[axis];
+    } else {
+        SFD_TRY(build_phase<P>(c, axis, tau, cD, Om, imag, &ph, &rows));
+    }
+    a.psi = psi;
+    a.V = V;
+    a.phase = ph;
+    a.phase_row_stride = rows > 1 ? c->n[axis] : 0;
+    if (axis == 0) a.row_mod = rows > 1 ? c->n[1] : 0;
+    else a.row_from_col = (axis == 1 && rows > 1) ? 1 : 0;
+    a.flags = flags | (imag ? kFlagImag : 0);
+    a.beta = beta;
+    a.dt = dt;
+    a.scale = ws ? ws->scal : c->scal;
+    a.save = save;
+    if (flags & kFlagNorm) {
+        if (ws) {
+            a.partial = ws->partial;
+        } else {
+            SFD_TRY(ensure_partial(c, 5 * sweep_nblocks(c, axis)));
+            a.partial = c->partial;
+        }
+    }
+    if (twice) {
+        if (epartial) {
+            if (imag) return fail(SFD_ERR_ARG, "no energy sums in an imaginary-time double round trip");
+            a.epartial = epartial;
+            a.xi = c->xi[axis];
+            return launch_sweep<P, kRoundTripTwiceEnergy>(c, axis, a);
+        }
+        if constexpr (sizeof(R) == 4)
+            if (imag) return launch_sweep<P, kRoundTripTwiceImag>(c, axis, a);
+        return launch_sweep<P, kRoundTripTwice>(c, axis, a);
+    }
+    if (epartial) {
+        a.epartial = epartial;
+        a.xi = c->xi[axis];
+        if constexpr (sizeof(R) == 4)
+            if (imag) return launch_sweep<P, kRoundTripImagEnergy>(c, axis, a);
+        return launch_sweep<P, kRoundTripEnergy>(c, axis, a);
+    }
+    if constexpr (sizeof(R) == 4)
+        if (imag) return launch_sweep<P, kRoundTripImag>(c, axis, a);
+    return launch_sweep<P, kRoundTrip>(c, axis, a);
+}
+
+template <typename P>
+int norm2_impl(sfd_ctx* c, const void* psi, double* dev_out)
+{
+    using S = typename P::S;
+    const long long nb = grid_blocks(c->nsite);
+    SFD_TRY(ensure_partial(c, nb));
+    hipLaunchKernelGGL(k_norm2<S>, dim3((unsigned)nb), dim3(256), 0, c->stream,
+                       static_cast<const cx<S>*>(psi), c->nsite, c->partial);
+    SFD_HIP(hipGetLastError());
+    launch_finalize(c->stream, c->partial, nb, 1, dev_out);
+    SFD_HIP(hipGetLastError());
+    return SFD_OK;
+}
+
+template <typename P>
+// nsteps split steps.  With `ehist` (device, 8 doubles per step), the first
+// half-step sweeps of step s also sum the energy of the state they load and
+// write the sums of that state (the state before step s) to ehist[8 s ..]:
+// [0] sum xi_x^2 |psi_hat_x|^2, [1] sum |psi|^2, [2] sum V|psi|^2,
+// [3] sum |psi|^4, [3 + d] sum xi_d^2 |psi_hat_d|^2 for d = 1, 2.  The y and
+// z sums are taken after the x (and y) flow, which leaves them unchanged
+// only in real time with Omega = 0 (the caller's condition).
+//
+// Strang steps run the axes as x, y, [z N z], x, y (fold): the innermost
+// axis's flow commutes with the others (its multiplier depends on its own
+// wavenumber only; with rotation the x and y flows also depend on y and x,
+// so this holds for z in 3D, and for y in 2D only when Omega = 0), so the
+// second half step K_z K_y K_x may apply z first, and the two z flows meet
+// around the nonlinear phase in one double round-trip sweep (2D - 1 sweeps
+// per step instead of 2D).  `fold = false` keeps the x, y, z, N, x, y, z
+// order of splitting.jl:149-154.
+int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt,
+               int64_t nsteps, int order, int mode, double* ehist = nullptr, bool fold = true,
+               const Workspace* ws = nullptr)
+{
+    using S = typename P::S;
+    const int imag = mode == SFD_IMAG_TIME;
+    const int D = c->ndim;
+    const double tau = order == 2 ? dt * 0.5 : dt;
+    const int nlflags = kFlagNL | (imag ? kFlagImag : 0);
+    // Reduction workspace for the per-step renormalisation (its last sweep
+    // is on axis D - 2 when folded, else on the last axis).
+    if (imag && !ws)
+        SFD_TRY(ensure_partial(c, 5 * std::max(sweep_nblocks(c, D >= 2 ? D - 2 : 0), sweep_nblocks(c, D - 1))));
+    double* scal = ws ? ws->scal : c->scal;
+    double* npart = ws ? ws->partial : c->partial;
+    // Energy partials: one region per axis, all finalized after the step's
+    // first half in one launch.
+    double* epart[3] = {nullptr, nullptr, nullptr};
+    if (ehist) {
+        long long off[3], tot = 0;
+        for (int ax = 0; ax < D; ++ax) {
+            off[ax] = tot;
+            tot += 4 * sweep_nblocks(c, ax);
+        }
+        SFD_TRY(ensure_buf(c, &c->epartial, &c->epartial_cap, tot));
+        for (int ax = 0; ax < D; ++ax) epart[ax] = c->epartial + off[ax];
+    }
+    bool pending = false;
+    fold = fold && order == 2 && D >= 2 && (Om == 0.0 || D == 3);
+    for (int64_t s = 0; s < nsteps && fold; ++s) {
+        FinalizeSets fs{};
+        auto record = [&](int ax) {
+            fs.partial[ax] = epart[ax];
+            fs.nb[ax] = c->last_nblocks;
+            fs.K[ax] = ax == 0 ? 4 : 1;
+            fs.out[ax] = ehist + 8 * s + (ax == 0 ? 0 : 3 + ax);
+        };
+        for (int ax = 0; ax < D - 1; ++ax) {
+            int flags = (ax == 0 && pending) ? kFlagScale : 0;
+            if (ehist && ax == 0) flags |= kFlagPointwise;
+            SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, flags, beta, dt, nullptr, epart[ax], false, ws));
+            if (ehist) record(ax);
+        }
+        SFD_TRY(round_trip<P>(c, psi, V, D - 1, tau, cD, Om, imag, nlflags, beta, dt, nullptr, epart[D - 1], true,
+                              ws));
+        if (ehist) {
+            record(D - 1);
+            launch_finalize_sets(c->stream, fs);
+            SFD_HIP(hipGetLastError());
+        }
+        for (int ax = 0; ax < D - 1; ++ax)
+            SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, (ax == D - 2 && imag) ? kFlagNorm : 0, beta,
+                                  dt, nullptr, nullptr, false, ws));
+        if (imag) {
+            launch_finalize(c->stream, npart, c->last_nblocks, 1, scal + 1, c->dvol(), scal);
+            SFD_HIP(hipGetLastError());
+            pending = true;
+        }
+    }
+    for (int64_t s = 0; s < nsteps && !fold; ++s) {
+        FinalizeSets fs{};
+        for (int h = 0; h < order; ++h) {
+            for (int ax = 0; ax < D; ++ax) {
+                int flags = 0;
+                if (ax == 0 && h == 0 && pending) flags |= kFlagScale;
+                if (ax == D - 1 && h == 0) flags |= nlflags;
+                if (ax == D - 1 && h == order - 1 && imag) flags |= kFlagNorm;
+                const bool en = ehist && h == 0;
+                if (en && ax == 0) flags |= kFlagPointwise;
+                SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, flags, beta, dt, nullptr,
+                                      en ? epart[ax] : nullptr, false, ws));
+                if (en) {
+                    fs.partial[ax] = epart[ax];
+                    fs.nb[ax] = c->last_nblocks;
+                    fs.K[ax] = ax == 0 ? 4 : 1;
+                    fs.out[ax] = ehist + 8 * s + (ax == 0 ? 0 : 3 + ax);
+                }
+            }
+            if (ehist && h == 0) {
+                launch_finalize_sets(c->stream, fs);
+                SFD_HIP(hipGetLastError());
+            }
+        }
+        if (imag) {
+            const long long nb = c->last_nblocks;  // partials of the norm sweep just launched
+            launch_finalize(c->stream, npart, nb, 1, scal + 1, c->dvol(), scal);
+            SFD_HIP(hipGetLastError());
+            pending = true;
+        }
+    }
+    if (pending) {
+        const long long nb = grid_blocks(c->nsite);
+        hipLaunchKernelGGL(k_scale<S>, dim3((unsigned)nb), dim3(256), 0, c->stream, static_cast<cx<S>*>(psi),
+                           c->nsite, 0.0, (const double*)scal, 0);
+        SFD_HIP(hipGetLastError());
+    }
+    return SFD_OK;
+}
+
+template <typename P>
+int energy_impl(sfd_ctx* c, const void* psi, const void* V, double cD, double beta, double Om, double* out5)
+{
+    double sums[3][5];
+    long long off[3], tot = 0;
+    for (int ax = 0; ax < c->ndim; ++ax) {
+        off[ax] = tot;
+        tot += 5 * sweep_nblocks(c, ax);
+    }
+    SFD_TRY(ensure_partial(c, tot));
+    FinalizeSets fs{};
+    for (int ax = 0; ax < c->ndim; ++ax) {
+        SweepArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.psi = const_cast<void*>(psi);
+        a.V = V;
+        a.xi = c->xi[ax];
+        a.flags = ax == 0 ? kFlagPointwise : 0;
+        if (ax == 0 && c->ndim >= 2) { a.coord = c->coord[1]; a.coord_mod = c->n[1]; }
+        if (ax == 1) a.coord = c->coord[0];
+        a.partial = c->partial + off[ax];
+        SFD_TRY(launch_sweep<P, kEnergy>(c, ax, a));
+        fs.partial[ax] = a.partial;
+        fs.nb[ax] = c->last_nblocks;
+        fs.K[ax] = 5;
+        fs.out[ax] = c->scal + 8 + 5 * ax;
+    }
+    launch_finalize_sets(c->stream, fs);
+    SFD_HIP(hipGetLastError());
+    SFD_HIP(hipMemcpyAsync(sums, c->scal + 8, 5 * c->ndim * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SFD_HIP(hipStreamSynchronize(c->stream));
+    const double kin[3] = {sums[0][0], c->ndim > 1 ? sums[1][0] : 0.0, c->ndim > 2 ? sums[2][0] : 0.0};
+    const double rot[2] = {sums[0][1], c->ndim > 1 ? sums[1][1] : 0.0};
+    const double pw[3] = {sums[0][2], sums[0][3], sums[0][4]};
+    combine_energy(c, kin, rot, pw, cD, beta, Om, out5);
+    return SFD_OK;
+}
+
+// solve!'s loop: nsteps of (energy of the state, split step).  In real time
+// with Omega = 0 the energy comes out of each step's first-half sweeps
+// (split_impl with a history): the y and z sums are taken after the x (and
+// y) flow, which is unitary along its axis and commutes with the y/z
+// transforms, so the sums are the state's own.  With rotation the x flow's
+// phase depends on y, and in imaginary time the flows damp; each step is
+// then preceded by a separate energy evaluation.  energies: host, 5 per step.
+template <typename P>
+int split_energy_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt,
+                      int64_t nsteps, int order, int mode, double* energies)
+{
+    if (Om != 0.0 || mode != SFD_REAL_TIME) {
+        for (int64_t s = 0; s < nsteps; ++s) {
+            SFD_TRY(energy_impl<P>(c, psi, V, cD, beta, Om, energies + 5 * s));
+            SFD_TRY(split_impl<P>(c, psi, V, cD, beta, Om, dt, 1, order, mode));
+        }
+        return SFD_OK;
+    }
+    constexpr int64_t kChunk = 4096;  // steps per history buffer (256 KiB)
+    std::vector<double> h;
+    for (int64_t s0 = 0; s0 < nsteps; s0 += kChunk) {
+        const int64_t m = std::min(kChunk, nsteps - s0);
+        SFD_TRY(ensure_buf(c, &c->ehist, &c->ehist_cap, 8 * m));
+        SFD_TRY(split_impl<P>(c, psi, V, cD, beta, Om, dt, m, order, mode, c->ehist));
+        h.resize(8 * (size_t)m);
+        SFD_HIP(hipMemcpyAsync(h.data(), c->ehist, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        SFD_HIP(hipStreamSynchronize(c->stream));
+        for (int64_t s = 0; s < m; ++s) {
+            const double* r = h.data() + 8 * s;
+            const double kin[3] = {r[0], r[4], r[5]};
+            const double rot[2] = {0.0, 0.0};
+            const double pw[3] = {r[1], r[2], r[3]};
+            combine_energy(c, kin, rot, pw, cD, beta, Om, energies + 5 * (s0 + s));
+        }
+    }
+    return SFD_OK;
+}
+
+template <typename P>
+int fft_impl(sfd_ctx* c, const void* in, void* out, int axis, int dir)
+{
+    SweepArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.psi = const_cast<void*>(in);
+    a.out = out;
+    if (dir == SFD_FORWARD) return launch_sweep<P, kForward>(c, axis, a);
+    return launch_sweep<P, kInverse>(c, axis, a);
+}
+
+// One real-time Strang step that also stores the state before the
+// nonlinear phase (psi1 = K psi) in `save`.  Where split_impl folds the step
+// (D >= 2, and Omega = 0 or D = 3) this runs the same launches, the double
+// round trip storing psi1, so that recomputed states equal the forward
+// pass's bit for bit.
+template <typename P>
+int strang_step_save(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt, void* save)
+{
+    const int D = c->ndim;
+    const double tau = 0.5 * dt;
+    if (D >= 2 && (Om == 0.0 || D == 3)) {
+        for (int ax = 0; ax < D - 1; ++ax) SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt));
+        SFD_TRY(round_trip<P>(c, psi, V, D - 1, tau, cD, Om, 0, kFlagNL, beta, dt, save, nullptr, true));
+        for (int ax = 0; ax < D - 1; ++ax) SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt));
+        return SFD_OK;
+    }
+    for (int h = 0; h < 2; ++h)
+        for (int ax = 0; ax < D; ++ax) {
+            const bool nl = (h == 0 && ax == D - 1);
+            SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, nl ? kFlagNL : 0, beta, dt, nl ? save : nullptr));
+        }
+    return SFD_OK;
+}
+
+// K^dagger: round trips over -tau in reverse axis order.
+template <typename P>
+int lap_rot_adjoint(sfd_ctx* c, void* G, double cD, double Om, double tau)
+{
+    for (int ax = c->ndim - 1; ax >= 0; --ax) SFD_TRY(round_trip<P>(c, G, nullptr, ax, -tau, cD, Om, 0, 0, 0.0, 0.0));
+    return SFD_OK;
+}
+
+// Gradient of L = 1 - |<target|psi_N>|^2 through nsteps real-time Strang
+// steps by the reverse sweep, recomputing forward states from checkpoints
+// taken every `every` steps (sqrt(nsteps) when every <= 0, at most nsteps):
+// with K = ceil(nsteps/every) checkpoints and S side streams (2 for fields of
+// at most 16 MiB, else 1; never more than K), memory is
+// K + min(S + 1, K) every + S + 2 fields, forward work twice the plain run.
+template <typename P>
+int adjoint_impl(sfd_ctx* c, const void* psi0, const void* V, const void* target, double cD, double beta, double Om,
+                 double dt, int64_t N, int64_t every, double* gradV, double* grad_beta, double* loss, void* psi_final)
+{
+    using S = typename P::S;
+    using R = typename P::R;
+    const long long ns = c->nsite;
+    const size_t fb = (size_t)ns * sizeof(cx<S>);
+    if (every <= 0) every = (int64_t)std::ceil(std::sqrt((double)(N > 0 ? N : 1)));
+    every = std::min<int64_t>(every, N > 0 ? N : 1);  // one segment at most covers the run
+    const int64_t nck = N > 0 ? (N + every - 1) / every : 0;
+    const long long nb = grid_blocks(ns);
+    // Small fields leave most CUs idle in every sweep: two side streams then
+    // recompute two segments at once.  No more streams than segments, and no
+    // more saved-state buffers than segments.
+    int nside = adjoint_side_streams(fb);
+    nside = (int)std::max<int64_t>(1, std::min<int64_t>(nside, nck));
+    const int nbuf = (int)std::max<int64_t>(1, std::min<int64_t>(nside + 1, nck));
+    size_t ck_bytes = 0, seg_fields = 0, seg_bytes = 0, psi_bytes = 0;
+    if (__builtin_mul_overflow(fb, (size_t)nck, &ck_bytes) ||
+        __builtin_mul_overflow((size_t)every, (size_t)nbuf, &seg_fields) ||
+        __builtin_mul_overflow(fb, seg_fields, &seg_bytes) || __builtin_mul_overflow(fb, (size_t)nside, &psi_bytes))
+        return fail(SFD_ERR_ARG, "sfd_adjoint: checkpoint storage for %lld steps every %lld overflows",
+                    (long long)N, (long long)every);
+    DevBuf psi, G, ck, seg, part, acc;
+    SFD_HIP(hipMalloc(&psi.p, psi_bytes));
+    SFD_HIP(hipMalloc(&G.p, fb));
+    if (nck > 0) SFD_HIP(hipMalloc(&ck.p, ck_bytes));
+    SFD_HIP(hipMalloc(&seg.p, seg_bytes));
+    SFD_HIP(hipMalloc(&part.p, 2 * (size_t)nb * sizeof(double)));
+    SFD_HIP(hipMalloc(&acc.p, 4 * sizeof(double)));
+    auto fld = [&](void* base, int64_t k) { return static_cast<void*>(static_cast<char*>(base) + fb * (size_t)k); };
+    double* accd = static_cast<double*>(acc.p);
+    double* partd = static_cast<double*>(part.p);
+
+    // forward pass with checkpoints
+    SFD_HIP(hipMemcpyAsync(psi.p, psi0, fb, hipMemcpyDeviceToDevice, c->stream));
+    for (int64_t k = 0; k < nck; ++k) {
+        SFD_HIP(hipMemcpyAsync(fld(ck.p, k), psi.p, fb, hipMemcpyDeviceToDevice, c->stream));
+        const int64_t m = std::min<int64_t>(every, N - k * every);
+        SFD_TRY(split_impl<P>(c, psi.p, V, cD, beta, Om, dt, m, 2, SFD_REAL_TIME, nullptr));
+    }
+    if (psi_final) SFD_HIP(hipMemcpyAsync(psi_final, psi.p, fb, hipMemcpyDeviceToDevice, c->stream));
+
+    // loss and terminal gradient
+    hipLaunchKernelGGL(k_inner<S>, dim3((unsigned)nb), dim3(256), 0, c->stream, static_cast<const cx<S>*>(target),
+                       static_cast<const cx<S>*>(psi.p), ns, partd);
+    SFD_HIP(hipGetLastError());
+    launch_finalize(c->stream, partd, nb, 2, accd);
+    SFD_HIP(hipGetLastError());
+    double cs[2];
+    SFD_HIP(hipMemcpyAsync(cs, accd, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SFD_HIP(hipStreamSynchronize(c->stream));
+    const double dv = c->dvol();
+    const double cr = cs[0] * dv, ci = cs[1] * dv;
+    if (loss) *loss = 1.0 - (cr * cr + ci * ci);
+    hipLaunchKernelGGL(k_terminal<S>, dim3((unsigned)nb), dim3(256), 0, c->stream, static_cast<cx<S>*>(G.p),
+                       static_cast<const cx<S>*>(target), ns, (const double*)accd, dv);
+    SFD_HIP(hipGetLastError());
+    if (gradV) SFD_HIP(hipMemsetAsync(gradV, 0, (size_t)ns * sizeof(double), c->stream));
+    SFD_HIP(hipMemsetAsync(accd + 2, 0, sizeof(double), c->stream));
+    // dL/dbeta: per-workgroup sums over all reversed steps, reduced once
+    SFD_HIP(hipMemsetAsync(partd, 0, (size_t)nb * sizeof(double), c->stream));
+
+    // reverse sweep, segment by segment from the last checkpoint.  With
+    // Omega = 0 the axis flows commute, so the trailing K^dagger(tau) of one
+    // reversed step and the leading one of the step before it run as one
+    // K^dagger(2 tau); the sequence of operations on G does not depend on
+    // the checkpoint interval.
+    const double tau = 0.5 * dt;
+    const bool merge = Om == 0.0;
+    // Segment i (checkpoint k = nck - 1 - i) is recomputed on side stream
+    // i % nside, up to nside segments ahead of the one reversed on the
+    // context's stream; its saved states go to buffer i % nbuf of `seg`.
+    // Every phase table the streams read is built first, and the cache
+    // evicts nothing from then on (the side streams' destructor lifts the pin).
+    SideStreams ss;
+    SFD_TRY(ss.init(c, nside));
+    c->pin_phases = true;
+    for (int ax = 0; ax < c->ndim; ++ax) {
+        void* ph = nullptr;
+        int rows = 0;
+        SFD_TRY(build_phase<P>(c, ax, tau, cD, Om, 0, &ph, &rows));
+        SFD_TRY(build_phase<P>(c, ax, -tau, cD, Om, 0, &ph, &rows));
+        if (merge) SFD_TRY(build_phase<P>(c, ax, -2.0 * tau, cD, Om, 0, &ph, &rows));
+    }
+    auto seg_len = [&](int64_t i) { return std::min<int64_t>(every, N - (nck - 1 - i) * every); };
+    auto saved = [&](int64_t i, int64_t j) { return fld(seg.p, (i % nbuf) * every + j); };
+    auto recompute = [&](int64_t i) -> int {
+        hipStream_t st = ss.side[i % nside];
+        void* work = fld(psi.p, i % nside);
+        if (i >= nbuf) SFD_HIP(hipStreamWaitEvent(st, ss.freed[i % nbuf], 0));  // segment i - nbuf reversed
+        c->stream = st;
+        SFD_HIP(hipMemcpyAsync(work, fld(ck.p, nck - 1 - i), fb, hipMemcpyDeviceToDevice, st));
+        for (int64_t j = 0; j < seg_len(i); ++j)
+            SFD_TRY(strang_step_save<P>(c, work, V, cD, beta, Om, dt, saved(i, j)));
+        c->stream = ss.main;
+        SFD_HIP(hipEventRecord(ss.ready[i % nbuf], st));
+        return SFD_OK;
+    };
+    int64_t queued = 0;  // segments whose recompute is enqueued
+    if (nck > 0) {
+        SFD_HIP(hipEventRecord(ss.start, ss.main));  // checkpoints written, psi free
+        for (int k = 0; k < nside; ++k) SFD_HIP(hipStreamWaitEvent(ss.side[k], ss.start, 0));
+    }
+    bool started = false;
+    for (int64_t i = 0; i < nck; ++i) {
+        for (; queued < nck && queued <= i + nside; ++queued) SFD_TRY(recompute(queued));
+        SFD_HIP(hipStreamWaitEvent(ss.main, ss.ready[i % nbuf], 0));
+        for (int64_t j = seg_len(i) - 1; j >= 0; --j) {
+            SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, (merge && started) ? 2.0 * tau : tau));
+            started = true;
+            hipLaunchKernelGGL((k_nl_adjoint<S, R>), dim3((unsigned)nb), dim3(256), 0, c->stream,
+                               static_cast<cx<S>*>(G.p), static_cast<const cx<S>*>(saved(i, j)),
+                               static_cast<const S*>(V), ns, (R)beta, (R)dt, gradV, partd);
+            SFD_HIP(hipGetLastError());
+            if (!merge) SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
+        }
+        SFD_HIP(hipEventRecord(ss.freed[i % nbuf], ss.main));
+    }
+    if (merge && started) SFD_TRY(lap_rot_adjoint<P>(c, G.p, cD, Om, tau));
+    hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(256), 0, c->stream, (const double*)partd, nb, accd + 2);
+    SFD_HIP(hipGetLastError());
+    double gb = 0.0;
+    SFD_HIP(hipMemcpyAsync(&gb, accd + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SFD_HIP(hipStreamSynchronize(c->stream));
+    if (grad_beta) *grad_beta = gb;
+    return SFD_OK;
+}
+
+// Autotuner: for every axis with more than one compiled variant, time the
+// identity round trip (tau = 0) of each variant on a scratch field and keep
+// the fastest.  Each candidate gets 2 warm-up launches, then the best of 3
+// timed batches of 8 launches (HIP events on the context's stream).
+template <typename P>
+int autotune_impl(sfd_ctx* c, void* scratch, float* times_out)
+{
+    using S = typename P::S;
+    void* buf = scratch;
+    bool own = false;
+    if (!buf) {
+        SFD_HIP(hipMalloc(&buf, (size_t)c->nsite * sizeof(cx<S>)));
+        own = true;
+    }
+    int rc = SFD_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(SFD_ERR_HIP, "hipEventCreate failed");
+    }
+    if (rc == SFD_OK) {
+        hipLaunchKernelGGL(k_fill_phases<S>, dim3((unsigned)grid_blocks(c->nsite)), dim3(256), 0, c->stream,
+                           static_cast<cx<S>*>(buf), c->nsite);
+        if (hipGetLastError() != hipSuccess) rc = fail(SFD_ERR_HIP, "scratch fill failed");
+    }
+    for (int ax = 0; rc == SFD_OK && ax < c->ndim; ++ax) {
+        const int nv = n_variants<P>(c->n[ax], ax == 0);
+        int best = 0;
+        float best_ms = 1e30f;
+        for (int v = 0; rc == SFD_OK && v < nv; ++v) {
+            if ((rc = set_variant<P>(c, ax, v)) != SFD_OK) break;
+            float vbest = 1e30f;
+            for (int w = 0; rc == SFD_OK && w < 2; ++w)
+                rc = round_trip<P>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, 0, 0.0, 0.0);
+            for (int rep = 0; rc == SFD_OK && rep < 3; ++rep) {
+                (void)hipEventRecord(e0, c->stream);
+                for (int k = 0; rc == SFD_OK && k < 8; ++k)
+                    rc = round_trip<P>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, 0, 0.0, 0.0);
+                (void)hipEventRecord(e1, c->stream);
+                if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(SFD_ERR_HIP, "event sync failed"); break; }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms / 8 < vbest) vbest = ms / 8;
+            }
+            if (times_out) times_out[ax * 8 + (v < 8 ? v : 7)] = vbest;
+            if (vbest < best_ms) { best_ms = vbest; best = v; }
+        }
+        if (rc == SFD_OK) rc = set_variant<P>(c, ax, best);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (own) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(buf);
+    }
+    if (rc == SFD_OK) c->tuned = 1;
+    return rc;
+}
+
+template <typename P>
+int init_tables(sfd_ctx* c)
+{
+    for (int ax = 0; ax < c->ndim; ++ax) {
+        const int n = c->n[ax];
+        SFD_TRY(set_variant<P>(c, ax, 0));
+        std::vector<double> xi(n), co(n);
+        for (int i = 0; i < n; ++i) {
+            xi[i] = fft_freq(n, c->d[ax], i);
+            co[i] = lin_coord(c->lo[ax], c->hi[ax], n, i);
+        }
+        SFD_HIP(hipMalloc(&c->xi[ax], n * sizeof(double)));
+        SFD_HIP(hipMalloc(&c->coord[ax], n * sizeof(double)));
+        SFD_TRY(upload_sync(c->xi[ax], xi.data(), n * sizeof(double)));
+        SFD_TRY(upload_sync(c->coord[ax], co.data(), n * sizeof(double)));
+    }
+    SFD_HIP(hipMalloc(&c->scal, 32 * sizeof(double)));
+    std::vector<double> one(32, 0.0);
+    one[0] = 1.0;
+    SFD_TRY(upload_sync(c->scal, one.data(), 32 * sizeof(double)));
+    return SFD_OK;
+}
+
+// Launch variant `idx` for one axis of a size group: its twiddles (rebuilt
+// when the line split changes) and tile count.
+template <typename P>
+int batch_group_variant(sfd_batch* b, sfd_batch::Group& G, int ax, int idx)
+{
+    using R = typename P::R;
+    VariantInfo vi;
+    if (!variant_info<P>(G.n[ax], ax == 0, idx, &vi))
+        return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", idx, G.n[ax]);
+    if (!G.tw[ax] || G.vi[ax].n1 != vi.n1 || G.vi[ax].n2 != vi.n2 || G.vi[ax].h != vi.h) {
+        std::vector<cx<R>> w;
+        fill_twiddles<R>(w, G.n[ax], vi.n1, vi.n2, vi.h);
+        if (G.tw[ax]) {
+            SFD_HIP(hipStreamSynchronize(b->stream));
+            SFD_HIP(hipFree(G.tw[ax]));
+            G.tw[ax] = nullptr;
+        }
+        SFD_HIP(hipMalloc(&G.tw[ax], w.size() * sizeof(cx<R>)));
+        SFD_TRY(upload_sync(G.tw[ax], w.data(), w.size() * sizeof(cx<R>)));
+    }
+    G.vi[ax] = vi;
+    G.var[ax] = idx;
+    const int NL = vi.ts * vi.g;
+    G.ntiles[ax] = ax == 0 ? (G.n[1] + NL - 1) / NL : (G.n[0] + NL - 1) / NL;
+    return SFD_OK;
+}
+
+template <typename P>
+int batch_setup(sfd_batch* b)
+{
+    using S = typename P::S;
+    // groups of equal extents, internal order group-major
+    std::vector<sfd_batch::Inst> sorted;
+    for (auto& in : b->inst) {
+        int gidx = -1;
+        for (size_t g = 0; g < b->groups.size(); ++g)
+            if (b->groups[g].n[0] == in.n[0] && b->groups[g].n[1] == in.n[1]) gidx = (int)g;
+        if (gidx < 0) {
+            sfd_batch::Group G;
+            G.n[0] = in.n[0];
+            G.n[1] = in.n[1];
+            b->groups.push_back(G);
+            gidx = (int)b->groups.size() - 1;
+        }
+        in.group = gidx;
+    }
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        b->groups[g].first = (int)sorted.size();
+        for (auto& in : b->inst)
+            if (in.group == (int)g) sorted.push_back(in);
+        b->groups[g].count = (int)sorted.size() - b->groups[g].first;
+    }
+    b->inst = sorted;
+    const int ni = b->ninst;
+    long long parts = 0, sites = 0;
+    for (auto& G : b->groups) {
+        for (int ax = 0; ax < 2; ++ax) SFD_TRY(batch_group_variant<P>(b, G, ax, 0));
+        // norm partials come from the last x sweep (one per tile of G.n[1]
+        // x lines): room for the variant with the most tiles, so the
+        // autotuner may switch variants
+        long long most = 0;
+        for (int v = 0; v < n_variants<P>(G.n[0], true); ++v) {
+            VariantInfo vi;
+            if (variant_info<P>(G.n[0], true, v, &vi)) {
+                const int NL = vi.ts * vi.g;
+                most = std::max(most, (long long)((G.n[1] + NL - 1) / NL));
+            }
+        }
+        parts += most * G.count;
+    }
+    for (auto& in : b->inst) sites += in.nsite;
+    SFD_HIP(hipMalloc(&b->d_inst, ni * sizeof(BatchInstDev)));
+    SFD_HIP(hipMalloc(&b->d_scale, ni * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_snap_scale, ni * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_dpsi, ni * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_done, ni * sizeof(int)));
+    SFD_HIP(hipMalloc(&b->d_steps, ni * sizeof(long long)));
+    SFD_HIP(hipMalloc(&b->d_partial, parts * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_dpart, (size_t)ni * sfd_batch::kDiffBlocks * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_snap, (size_t)sites * sizeof(cx<S>)));
+    SFD_HIP(hipMalloc(&b->d_desc, sfd_batch::kKinds * (size_t)ni * sizeof(SweepArgs)));
+    b->phase.assign(2 * (size_t)ni, nullptr);
+    SFD_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        SFD_HIP(hipStreamCreate(&st));
+        b->gstream.push_back(st);
+        SFD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        b->gdone.push_back(ev);
+    }
+    return SFD_OK;
+}
+
+// Descriptors and phase tables of one solve (fields, potentials, beta, dt).
+template <typename P>
+int batch_bind(sfd_batch* b, void* const* psi, const void* const* V, const double* beta, double cD, double dt)
+{
+    using R = typename P::R;
+    using S = typename P::S;
+    const int ni = b->ninst;
+    std::vector<BatchInstDev> hi(ni);
+    std::vector<SweepArgs> desc(sfd_batch::kKinds * (size_t)ni);
+    long long poff = 0, soff = 0;
+    const double tau = 0.5 * dt;
+    for (int i = 0; i < ni; ++i) {
+        const auto& in = b->inst[i];
+        const auto& G = b->groups[in.group];
+        if (!psi[in.user]) return fail(SFD_ERR_NULL, "batch: field %d is null", in.user);
+        hi[i].psi = psi[in.user];
+        hi[i].snap = static_cast<char*>(b->d_snap) + soff * sizeof(cx<S>);
+        hi[i].partial = b->d_partial + poff;
+        hi[i].nsite = in.nsite;
+        hi[i].nparts = G.ntiles[0];
+        hi[i].dv = in.d[0] * in.d[1];
+        // imaginary-time multipliers exp(cD xi^2 tau)/n of both axes
+        for (int ax = 0; ax < 2; ++ax) {
+            const int n = in.n[ax];
+            std::vector<cx<R>> h(n);
+            for (int k = 0; k < n; ++k) {
+                const double xi = fft_freq(n, in.d[ax], k);
+                h[k] = decay_entry<R>((cD * (xi * xi)) * tau, 1.0 / (double)n);
+            }
+            void*& dev = b->phase[2 * (size_t)i + ax];
+            if (!dev) SFD_HIP(hipMalloc(&dev, n * sizeof(cx<R>)));
+            SFD_TRY(upload_sync(dev, h.data(), n * sizeof(cx<R>)));
+        }
+        for (int k = 0; k < sfd_batch::kKinds; ++k) {
+            const int ax = sfd_batch::kind_axis(k);
+            SweepArgs a;
+            std::memset(&a, 0, sizeof a);
+            a.psi = psi[in.user];
+            a.V = V ? V[in.user] : nullptr;
+            a.phase = b->phase[2 * (size_t)i + ax];
+            a.tw = G.tw[ax];
+            a.scale = b->d_scale + i;
+            a.beta = beta[in.user];
+            a.dt = dt;
+            batch_geom(a, in, G, ax);
+            // x (rescale), [y, nonlinear, y], x (+ norm partials): the
+            // folded Strang step of split_impl
+            a.flags = kFlagImag;
+            if (k == 0) a.flags |= kFlagScale;
+            if (k == 1) a.flags |= kFlagNL;
+            if (k == 2) {
+                a.flags |= kFlagNorm;
+                a.partial = b->d_partial + poff;
+            }
+            desc[(size_t)k * ni + i] = a;
+        }
+        poff += G.ntiles[0];
+        soff += in.nsite;
+    }
+    SFD_TRY(upload_sync(b->d_inst, hi.data(), hi.size() * sizeof(BatchInstDev)));
+    SFD_TRY(upload_sync(b->d_desc, desc.data(), desc.size() * sizeof(SweepArgs)));
+    return SFD_OK;
+}
+
+// Enqueue k imaginary-time Strang steps of every unfinished instance; the
+// last step is a convergence check when `check` is set.  Each size group
+// runs its k steps on its own stream (forked from and joined back to the
+// batch stream), so the groups' small launches overlap.
+template <typename P>
+int batch_enqueue(sfd_batch* b, int k, bool check, double tol)
+{
+    using S = typename P::S;
+    const int ni = b->ninst;
+    constexpr int KB = sfd_batch::kDiffBlocks;
+    SFD_HIP(hipEventRecord(b->fork, b->stream));
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        const auto& G = b->groups[gi];
+        hipStream_t st = b->gstream[gi];
+        SFD_HIP(hipStreamWaitEvent(st, b->fork, 0));
+        const int f = G.first;
+        for (int s = 0; s < k; ++s) {
+            const bool chk = check && s == k - 1;
+            if (chk) {
+                hipLaunchKernelGGL(k_batch_snapshot<S>, dim3(KB, G.count), dim3(256), 0, st, b->d_inst + f,
+                                   b->d_scale + f, b->d_snap_scale + f, b->d_done + f);
+                SFD_HIP(hipGetLastError());
+            }
+            for (int kind = 0; kind < sfd_batch::kKinds; ++kind) {
+                const int ax = sfd_batch::kind_axis(kind);
+                const VariantInfo& vi = G.vi[ax];
+                dim3 grid((unsigned)G.ntiles[ax], 1, (unsigned)G.count);
+                dim3 block(vi.ts * vi.g * vi.n2 * vi.h);
+                if (!launch_batch<P>(G.n[ax], ax == 0, G.var[ax], grid, block, st,
+                                     b->d_desc + (size_t)kind * ni + f, b->d_done + f, kind == 1))
+                    return fail(SFD_ERR_SHAPE, "batch: no kernel for line length %d", G.n[ax]);
+                SFD_HIP(hipGetLastError());
+            }
+            hipLaunchKernelGGL(k_batch_finalize, dim3(G.count), dim3(256), 0, st, b->d_inst + f, b->d_scale + f,
+                               b->d_done + f, b->d_steps + f);
+            SFD_HIP(hipGetLastError());
+            if (chk) {
+                hipLaunchKernelGGL(k_batch_diff<S>, dim3(KB, G.count), dim3(256), 0, st, b->d_inst + f,
+                                   b->d_scale + f, b->d_snap_scale + f, b->d_done + f, b->d_dpart + (size_t)f * KB);
+                SFD_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_batch_diff_finalize, dim3(G.count), dim3(256), 0, st,
+                                   b->d_dpart + (size_t)f * KB, KB, b->d_inst + f, b->d_dpsi + f, b->d_done + f,
+                                   tol);
+                SFD_HIP(hipGetLastError());
+            }
+        }
+        SFD_HIP(hipEventRecord(b->gdone[gi], st));
+        SFD_HIP(hipStreamWaitEvent(b->stream, b->gdone[gi], 0));
+    }
+    return SFD_OK;
+}
+
+// Batch autotuner: for every size group and axis with more than one compiled
+// variant, time the identity round trip (tau = 0) of k_batch_sweep (the
+// double round trip on y, as in a step) over the
+// group's instances on the snapshot buffer filled with unit-modulus
+// pseudo-random data (the fields are not touched), 2 warm-up launches then
+// the best of 3 batches of 8, and keep the
+// fastest.  times[16 g + 8 ax + v] = ms per launch of variant v (-1 if none).
+template <typename P>
+int batch_autotune_impl(sfd_batch* b, float* times)
+{
+    using R = typename P::R;
+    using S = typename P::S;
+    const int ni = b->ninst;
+    if (times)
+        for (int i = 0; i < 16 * ni; ++i) times[i] = -1.0f;
+    batch_drop_graph(b);
+    long long sites = 0;
+    for (auto& in : b->inst) sites += in.nsite;
+    SFD_HIP(hipEventRecord(b->ev, b->user_stream));
+    SFD_HIP(hipStreamWaitEvent(b->stream, b->ev, 0));
+    hipLaunchKernelGGL(k_fill_phases<S>, dim3((unsigned)grid_blocks(sites)), dim3(256), 0, b->stream,
+                       static_cast<cx<S>*>(b->d_snap), sites);
+    SFD_HIP(hipGetLastError());
+    SFD_HIP(hipMemsetAsync(b->d_done, 0, ni * sizeof(int), b->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<void*> tmp;  // identity phase tables
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(b->stream);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        for (void* p : tmp) (void)hipFree(p);
+    };
+    auto run = [&]() -> int {
+        SFD_HIP(hipEventCreate(&e0));
+        SFD_HIP(hipEventCreate(&e1));
+        for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+            auto& G = b->groups[gi];
+            for (int ax = 0; ax < 2; ++ax) {
+                const int nv = n_variants<P>(G.n[ax], ax == 0);
+                if (nv <= 1) continue;
+                const int n = G.n[ax];
+                std::vector<cx<R>> h(n, decay_entry<R>(0.0, 1.0 / (double)n));
+                void* ph = nullptr;
+                SFD_HIP(hipMalloc(&ph, n * sizeof(cx<R>)));
+                tmp.push_back(ph);
+                SFD_TRY(upload_sync(ph, h.data(), n * sizeof(cx<R>)));
+                float best = 1e30f;
+                int best_v = 0;
+                for (int v = 0; v < nv && v < 8; ++v) {
+                    SFD_TRY(batch_group_variant<P>(b, G, ax, v));
+                    std::vector<SweepArgs> desc(G.count);
+                    long long soff = 0;
+                    for (int i = 0; i < G.first; ++i) soff += b->inst[i].nsite;
+                    for (int k = 0; k < G.count; ++k) {
+                        const auto& in = b->inst[G.first + k];
+                        SweepArgs& a = desc[k];
+                        std::memset(&a, 0, sizeof a);
+                        a.psi = static_cast<char*>(b->d_snap) + soff * sizeof(cx<S>);
+                        a.phase = ph;
+                        a.tw = G.tw[ax];
+                        a.flags = kFlagImag;
+                        batch_geom(a, in, G, ax);
+                        soff += in.nsite;
+                    }
+                    SFD_TRY(upload_sync(b->d_desc, desc.data(), desc.size() * sizeof(SweepArgs)));
+                    const VariantInfo& vi = G.vi[ax];
+                    dim3 grid((unsigned)G.ntiles[ax], 1, (unsigned)G.count);
+                    dim3 block(vi.ts * vi.g * vi.n2 * vi.h);
+                    // the y sweep of a step is the double round trip
+                    auto launch = [&]() {
+                        return launch_batch<P>(n, ax == 0, v, grid, block, b->stream, b->d_desc, b->d_done, ax == 1);
+                    };
+                    for (int w = 0; w < 2; ++w)
+                        if (!launch()) return fail(SFD_ERR_SHAPE, "batch autotune: no kernel %d for n = %d", v, n);
+                    SFD_HIP(hipGetLastError());
+                    float t_best = 1e30f;
+                    for (int rep = 0; rep < 3; ++rep) {
+                        SFD_HIP(hipEventRecord(e0, b->stream));
+                        for (int k = 0; k < 8; ++k) launch();
+                        SFD_HIP(hipEventRecord(e1, b->stream));
+                        SFD_HIP(hipEventSynchronize(e1));
+                        float ms = 0.0f;
+                        SFD_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        t_best = std::min(t_best, ms / 8.0f);
+                    }
+                    if (times) times[16 * gi + 8 * ax + v] = t_best;
+                    if (t_best < best) {
+                        best = t_best;
+                        best_v = v;
+                    }
+                }
+                SFD_TRY(batch_group_variant<P>(b, G, ax, best_v));
+            }
+        }
+        return SFD_OK;
+    };
+    const int rc = run();
+    cleanup();
+    return rc;
+}
+
+template <typename P>
+int batch_run(sfd_batch* b, void* const* psi, const void* const* V, const double* beta, double cD, double dt,
+              int64_t max_steps, double tol, int check_every, int use_graph, int64_t* steps_out, double* dpsi_out)
+{
+    using S = typename P::S;
+    const int ni = b->ninst;
+    SFD_TRY(batch_bind<P>(b, psi, V, beta, cD, dt));
+    // order after the caller's pending work
+    SFD_HIP(hipEventRecord(b->ev, b->user_stream));
+    SFD_HIP(hipStreamWaitEvent(b->stream, b->ev, 0));
+    hipLaunchKernelGGL(k_batch_reset, dim3((ni + 255) / 256), dim3(256), 0, b->stream, b->d_scale, b->d_done,
+                       b->d_steps, b->d_dpsi, ni, 1);
+    SFD_HIP(hipGetLastError());
+    batch_drop_graph(b);
+    std::vector<int> done(ni, 0);
+    int64_t left = max_steps;
+    while (left > 0) {
+        const int k = (int)(left < check_every ? left : check_every);
+        if (use_graph && k == check_every && k > 0) {
+            if (!b->gexec) {
+                SFD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                int rc = batch_enqueue<P>(b, k, true, tol);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(b->stream, &g);
+                if (rc != SFD_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (e != hipSuccess) return fail(SFD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+                b->graph = g;
+                SFD_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+                b->graph_k = k;
+            }
+            SFD_HIP(hipGraphLaunch(b->gexec, b->stream));
+        } else {
+            SFD_TRY(batch_enqueue<P>(b, k, true, tol));
+        }
+        left -= k;
+        SFD_HIP(hipMemcpyAsync(done.data(), b->d_done, ni * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        SFD_HIP(hipStreamSynchronize(b->stream));
+        bool all = true;
+        for (int i = 0; i < ni; ++i) all = all && done[i];
+        if (all) break;
+    }
+    // apply the pending renormalisation, leave normalised fields
+    hipLaunchKernelGGL(k_batch_apply_scale<S>, dim3(sfd_batch::kDiffBlocks, ni), dim3(256), 0, b->stream, b->d_inst,
+                       b->d_scale);
+    SFD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_batch_reset, dim3((ni + 255) / 256), dim3(256), 0, b->stream, b->d_scale, b->d_done,
+                       b->d_steps, b->d_dpsi, ni, 0);
+    SFD_HIP(hipGetLastError());
+    std::vector<long long> st(ni);
+    std::vector<double> dp(ni);
+    SFD_HIP(hipMemcpyAsync(st.data(), b->d_steps, ni * sizeof(long long), hipMemcpyDeviceToHost, b->stream));
+    SFD_HIP(hipMemcpyAsync(dp.data(), b->d_dpsi, ni * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    SFD_HIP(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < ni; ++i) {
+        if (steps_out) steps_out[b->inst[i].user] = st[i];
+        if (dpsi_out) dpsi_out[b->inst[i].user] = dp[i];
+    }
+    return SFD_OK;
+}
+
+// Wrappers with the signatures of PrecOps.
+template <typename P>
+int round_trip_axis(sfd_ctx* c, void* psi, int axis, double tau, double cD, double Om, int imag)
+{
+    return round_trip<P>(c, psi, nullptr, axis, tau, cD, Om, imag, 0, 0.0, 0.0);
+}
+
+template <typename P>
+int split_op(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt, int64_t nsteps,
+             int order, int mode, double* ehist, bool fold, const Workspace* ws)
+{
+    return split_impl<P>(c, psi, V, cD, beta, Om, dt, nsteps, order, mode, ehist, fold, ws);
+}
+
+template <typename P>
+PrecOps make_ops()
+{
+    PrecOps o;
+    o.init_tables = &init_tables<P>;
+    o.split = &split_op<P>;
+    o.split_energy = &split_energy_impl<P>;
+    o.round_trip_axis = &round_trip_axis<P>;
+    o.energy = &energy_impl<P>;
+    o.fft = &fft_impl<P>;
+    o.norm2 = &norm2_impl<P>;
+    o.adjoint = &adjoint_impl<P>;
+    o.autotune = &autotune_impl<P>;
+    o.set_variant = &set_variant<P>;
+    o.n_variants = &n_variants<P>;
+    o.build_phase_into = &launch_build_phase<P>;
+    o.table_entry_bytes = sizeof(cx<typename P::R>);
+    o.batch_setup = &batch_setup<P>;
+    o.batch_autotune = &batch_autotune_impl<P>;
+    o.batch_run = &batch_run<P>;
+    return o;
+}
+
+}  // namespace sfd

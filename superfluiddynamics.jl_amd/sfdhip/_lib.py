@@ -35,7 +35,7 @@ EXPORTS = (
     "sfd_upload", "sfd_download", "sfd_synchronize", "sfd_last_error", "sfd_version",
     "sfd_autotune", "sfd_ctx_get_variant", "sfd_ctx_set_variant",
     "sfd_batch_create", "sfd_batch_destroy", "sfd_batch_imag_run", "sfd_batch_autotune", "sfd_adjoint",
-    "sfd_project", "sfd_inner",
+    "sfd_project", "sfd_inner", "sfd_ctx_time_sweeps", "sfd_ctx_sweep_times",
 )
 
 
@@ -61,7 +61,7 @@ _lib = None
 
 def build(verbose: bool = False) -> str:
     """Compile libsfdhip.so for gfx950 with hipcc (Makefile in csrc/)."""
-    subprocess.run(["make", "-s", "-B", "-C", CSRC], check=True,
+    subprocess.run(["make", "-s", "-B", "-j4", "-C", CSRC], check=True,
                    stdout=None if verbose else subprocess.DEVNULL)
     return LIB_PATH
 
@@ -106,6 +106,8 @@ def load():
         "sfd_adjoint": ([vp, vp, vp, vp, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_i64, vp, dp, dp, vp], c_int),
         "sfd_project": ([vp, vp, vp, c_int, dp], c_int),
         "sfd_inner": ([vp, vp, vp, dp], c_int),
+        "sfd_ctx_time_sweeps": ([vp, c_int], c_int),
+        "sfd_ctx_sweep_times": ([vp, dp, i64p, c_int], c_int),
         "sfd_batch_imag_run": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), dp, c_dbl, c_dbl, c_i64, c_dbl, c_int,
                                 c_int, i64p, dp], c_int),
     }

@@ -279,6 +279,22 @@ def _real_dtype(f: Field):
     return torch.float32 if f.g.FT is np.float32 else torch.float64
 
 
+def _check_device_tensor(t, name: str, dtype, shape):
+    """Raise unless ``t`` is a contiguous HIP-device tensor of ``dtype`` and
+    ``shape``: its raw pointer goes straight to the kernels, which would
+    read a host pointer, or past the end of a smaller or narrower buffer."""
+    if not torch.is_tensor(t):
+        raise L.ArgError(L.SFD_ERR_ARG, f"{name} must be a torch tensor on the HIP device")
+    if t.device.type != "cuda":
+        raise L.ArgError(L.SFD_ERR_ARG, f"{name} is on {t.device}; it must be on the HIP device")
+    if t.dtype != dtype:
+        raise L.ArgError(L.SFD_ERR_ARG, f"{name} has dtype {t.dtype}; {dtype} is required")
+    if tuple(t.shape) != tuple(shape):
+        raise L.ShapeError(L.SFD_ERR_SHAPE, f"{name} has shape {tuple(t.shape)}; {tuple(shape)} is required")
+    if not t.is_contiguous():
+        raise L.ArgError(L.SFD_ERR_ARG, f"{name} must be contiguous")
+
+
 class PotentialQuadratic:
     """V = ½(1-α)(γx x² + γy y² [+ γz z²]) (quadratic.jl:61-67), evaluated in
     FP64 with the reference's operation order and rounded once to FT."""
@@ -478,6 +494,8 @@ def solve(n: _SplitBase, istart: int = 1, verbose: bool = False, writers=()):
 
 # ------------------------------------------------------ directional FFTs
 def _fft(f: Field, out, src, axis: int, direction: int):
+    _check_device_tensor(out, "out", f.ϕ.dtype, f.g.shape)
+    _check_device_tensor(src, "src", f.ϕ.dtype, f.g.shape)
     c = f.ctx()
     L.check(c.lib.sfd_fft_axis(c.h, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(out.data_ptr()),
                                axis, direction))
@@ -641,6 +659,8 @@ class BatchImagSolver:
 # ------------------------------------------------------------- adjoint
 def inner(f: Field, a, b) -> complex:
     """<a|b> = Σ conj(a) b ΠΔ (FP64) for tensors on f's grid."""
+    _check_device_tensor(a, "a", f.ϕ.dtype, f.g.shape)
+    _check_device_tensor(b, "b", f.ϕ.dtype, f.g.shape)
     c = f.ctx()
     out = (ctypes.c_double * 2)()
     L.check(c.lib.sfd_inner(c.h, ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), out))
@@ -654,6 +674,7 @@ def _adjoint_call(n: _SplitBase, Vptr, beta: float, target, nsteps: int, checkpo
     c = f.ctx()
     p = n.param
     tgt = target.ϕ if isinstance(target, Field) else target
+    _check_device_tensor(tgt, "target", f.ϕ.dtype, f.g.shape)
     gV = torch.zeros(f.g.shape, dtype=torch.float64, device="cuda")
     gb, loss = ctypes.c_double(), ctypes.c_double()
     out = torch.empty_like(f.ϕ) if want_state else None
@@ -715,10 +736,12 @@ def adjoint_gradient(n: _SplitBase, target, nsteps: int, checkpoint_every: int =
 def project(f: Field, field, basis) -> np.ndarray:
     """out[j] = Σ_x field[x]·basis[j, x] (FP64 device tensors; basis shape
     (nb,) + grid shape): the chain rule from dL/dV to V = Σ_j a_j basis_j."""
-    c = f.ctx()
-    field = field.to(torch.float64).contiguous()
-    basis = basis.to(torch.float64).contiguous()
+    _check_device_tensor(field, "field", torch.float64, f.g.shape)
+    if not torch.is_tensor(basis) or basis.dim() != len(f.g.shape) + 1:
+        raise L.ShapeError(L.SFD_ERR_SHAPE, f"basis must have shape (nb,) + {tuple(f.g.shape)}")
     nb = basis.shape[0]
+    _check_device_tensor(basis, "basis", torch.float64, (nb,) + tuple(f.g.shape))
+    c = f.ctx()
     out = (ctypes.c_double * nb)()
     L.check(c.lib.sfd_project(c.h, ctypes.c_void_p(field.data_ptr()), ctypes.c_void_p(basis.data_ptr()), nb, out))
     return np.array(out[:])

@@ -21,20 +21,29 @@ def test_bytes_per_site_update(prec, ndim, with_v, passes, want):
     assert bench.bytes_per_site_update(prec, ndim, with_v, passes) == want
 
 
-def test_committed_traffic_is_close_to_the_algorithmic_bytes():
-    """The calibrated counter bytes of the dominant sweep sit within 1 % above
-    one read and one write of the 256³ complex FP32 field (no re-reads)."""
-    t = bench.load_traffic("traffic_sweep_strided_f32_256.json")
+@pytest.mark.parametrize("slot", ["x round trip", "y round trip", "z round trip", "double round trip [d N d]"])
+def test_committed_traffic_is_close_to_the_algorithmic_bytes(slot):
+    """The calibrated counter bytes of every kernel of the step sit within 1 %
+    above one read and one write of the 256³ complex FP32 field (no re-reads)."""
+    t = bench.load_traffic(slot)
     assert t is not None
     alg = 2 * 8 * 256 ** 3
     assert alg <= t < 1.01 * alg
 
 
 def test_traffic_summary_is_plain_json():
-    path = os.path.join(bench.ROOT, "profiles", "traffic_sweep_strided_f32_256.json")
+    path = os.path.join(bench.ROOT, "profiles", "traffic_256cube_f32.json")
     with open(path) as fh:
         d = json.load(fh)
-    assert "traffic_bytes_per_launch" in d
+    assert d["algorithmic_bytes_per_launch"] == 2 * 8 * 256 ** 3
+    assert bench.load_traffic("other sweeps") is None
+
+
+def test_kernel_names_of_the_sweep_slots():
+    v = [(16, 16, 1, 8, 1, 1, 1), (16, 16, 16, 1, 1, 1, 1), (16, 16, 16, 1, 0, 1, 1)]
+    assert bench.kernel_name(v, 0) == "k_sweep_pipe<float, float, 16, 16, 1, 8>"
+    assert bench.kernel_name(v, 2) == "k_sweep<float, float, 16, 16, 16, 1>"
+    assert bench.kernel_name(v, 3) == "k_sweep<float, float, 16, 16, 16, 1, 7>"
 
 
 def test_more_than_one_gpu_is_refused(capsys):

@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     lib = sfdhip.load()
     for name in _declared():
         assert hasattr(lib, name), name
-    assert lib.sfd_version() == 100
+    assert lib.sfd_version() == 101
     assert lib.sfd_last_error() is not None
 
 

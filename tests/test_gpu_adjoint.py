@@ -183,3 +183,64 @@ def test_autograd_function_matches_adjoint_and_finite_differences(gpu, orc):
             print(f"a_{j}: autograd {a.grad[j].item():+.10e} fd {fd:+.10e}")
             assert abs(a.grad[j].item() - fd) <= 1e-6 * abs(fd)
     assert rel_l2(f.to_numpy(), psi0) < 1e-15  # the model's field is left unchanged
+
+
+def test_huge_checkpoint_interval_is_clamped_or_rejected(gpu):
+    """checkpoint_every is clamped to nsteps (one segment, nsteps saved states,
+    not three times that), and a step count whose checkpoint storage would
+    overflow size_t is rejected with SFD_ERR_ARG before any allocation."""
+    shape, bounds = (32, 16), ((-3.0, 3.0), (-3.0, 3.0))
+    psi0 = smooth_state(shape, bounds, seed=4)
+    tgt = smooth_state(shape, bounds, seed=5)
+    f, m = _model(gpu, psi0, bounds, random_potential(shape, bounds), 5.0)
+    t = torch.as_tensor(tgt).to("cuda")
+    ref = gpu.adjoint_gradient(m, t, 12, checkpoint_every=12)
+    for every in (13, 1 << 40, (1 << 62)):
+        l, gV, gb = gpu.adjoint_gradient(m, t, 12, checkpoint_every=every)
+        assert l == ref[0] and gb == ref[2] and torch.equal(gV, ref[1])
+    from sfdhip import _lib as L
+    with pytest.raises(L.ArgError, match="overflows"):
+        gpu.adjoint_gradient(m, t, 1 << 62, checkpoint_every=1 << 62)
+
+
+@pytest.mark.parametrize("nside", ["1", "2"])
+def test_side_stream_count_does_not_change_the_gradient(gpu, orc, nside, monkeypatch):
+    """The recompute runs on one or two side streams (forced through
+    SFD_ADJOINT_SIDE_STREAMS): results are bitwise those of the default and
+    equal the oracle adjoint, for several checkpoint intervals."""
+    shape, bounds = (32, 16), ((-3.0, 3.0), (-3.0, 3.0))
+    psi0 = smooth_state(shape, bounds, seed=1)
+    psi0 = psi0 / orc.norm(psi0, bounds)
+    tgt = smooth_state(shape, bounds, seed=8)
+    tgt = tgt / orc.norm(tgt, bounds)
+    V = random_potential(shape, bounds, amp=3.0)
+    f, m = _model(gpu, psi0, bounds, V, 6.0, 0.4)
+    t = torch.as_tensor(np.ascontiguousarray(tgt)).to("cuda")
+    monkeypatch.delenv("SFD_ADJOINT_SIDE_STREAMS", raising=False)
+    ref = {e: gpu.adjoint_gradient(m, t, 17, checkpoint_every=e) for e in (1, 3, 5, 0)}
+    monkeypatch.setenv("SFD_ADJOINT_SIDE_STREAMS", nside)
+    for e, r in ref.items():
+        l, gV, gb = gpu.adjoint_gradient(m, t, 17, checkpoint_every=e)
+        assert l == r[0] and gb == r[2] and torch.equal(gV, r[1])
+    loss, gVo, gbo = orc.adjoint(psi0, tgt, bounds, V=V, coeffD=-0.5, beta=6.0, Omega=0.4, dt=2e-3, nsteps=17)
+    assert gb == pytest.approx(gbo, rel=1e-10)
+    assert np.sqrt(np.sum((gV.cpu().numpy() - gVo) ** 2) / np.sum(gVo ** 2)) < 1e-10
+
+
+def test_full_phase_cache_does_not_change_the_gradient(gpu):
+    """With the phase-table cache already holding 32 unrelated tables, the
+    adjoint builds its own tables while the cache is pinned and the result is
+    bitwise unchanged."""
+    shape, bounds = (32, 32), ((-3.0, 3.0), (-3.0, 3.0))
+    psi0 = smooth_state(shape, bounds, seed=6)
+    tgt = smooth_state(shape, bounds, seed=7)
+    f, m = _model(gpu, psi0, bounds, random_potential(shape, bounds), 4.0, 0.2)
+    t = torch.as_tensor(tgt).to("cuda")
+    ref = gpu.adjoint_gradient(m, t, 9, checkpoint_every=2)
+    for k in range(40):  # 40 distinct taus x 2 axes on the same context: the cache is full of other tables
+        gpu.solveLapRot(m, 1e-3 * (k + 1))
+    f.set(psi0)
+    l, gV, gb = gpu.adjoint_gradient(m, t, 9, checkpoint_every=2)
+    assert l == ref[0] and gb == ref[2] and torch.equal(gV, ref[1])
+    l, gV, gb = gpu.adjoint_gradient(m, t, 9, checkpoint_every=2)
+    assert l == ref[0] and gb == ref[2] and torch.equal(gV, ref[1])
