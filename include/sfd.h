@@ -69,6 +69,20 @@ int sfd_ctx_set_stream(sfd_ctx* ctx, void* stream);
 int sfd_split_step(sfd_ctx* ctx, void* psi, const void* V, double coeffD, double beta,
                    double Omega, double dt, int64_t nsteps, int order, int mode);
 
+/* sfd_split_step without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments,
+ * kernels and results as sfd_split_step; the phase tables of the call, the
+ * renormalisation sums and partials live in a static 16 MiB device array of
+ * the library (one per GPU, allocated with the code object, held until the
+ * process ends) instead of the context's lazily allocated cache and buffers,
+ * and are rebuilt on every call.  Grids whose tables and partials exceed
+ * 16 MiB (e.g. rotating FP64 grids with nx*ny of 2^19 or more) return SFD_ERR_ARG.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (two streams, two host threads) race and must be ordered by the caller. */
+int sfd_split_step_noalloc(sfd_ctx* ctx, void* psi, const void* V, double coeffD, double beta,
+                           double Omega, double dt, int64_t nsteps, int order, int mode);
+
 /* solveLapRot!(n, tau) (splitting.jl:156-203): the linear flow over tau,
  * one round trip per axis in the order x, y[, z].  mode SFD_IMAG_TIME
  * replaces exp(i ...) by exp(...) (no renormalisation here). */

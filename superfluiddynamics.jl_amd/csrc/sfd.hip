@@ -10,10 +10,47 @@
 // precision (sfd_f32.hip, sfd_f64.hip, sfd_f32x.hip).
 #include "sfd_runtime.hpp"
 
+#include <mutex>
+
+// Device memory of sfd_split_step_noalloc: the pending scale and norm sums,
+// the norm partials of the renormalising sweep and the phase tables of the
+// call, in one statically sized array of every device the library runs on
+// (held until the process ends).  Host code never uses its name as an
+// address: each device's address comes from hipGetSymbolAddress once, at the
+// first sfd_ctx_create on that device, and is kept in g_scratch_ptr.
+static constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace sfd {
 
 namespace {
 thread_local std::string g_last_error;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+bool g_scratch_tried[kMaxDevices] = {};
+
+// Looks the scratch array up on the current device (once per device).
+void scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_tried[device]) return;
+    g_scratch_tried[device] = true;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[device] = p;
+    else (void)hipGetLastError();  // leave the device without scratch; the entry point reports it
+}
+
+void* scratch_ptr(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 }  // namespace
 
 int fail(int code, const char* fmt, ...)
@@ -364,6 +401,7 @@ int sfd_ctx_create(sfd_ctx** out, int ndim, const int64_t* n, const double* box_
     if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
         c->ncu < 1)
         c->ncu = 1;
+    scratch_setup(c->device);
     int rc = ops_for(c->prec).init_tables(c);
     if (rc != SFD_OK) {
         release(c);
@@ -728,6 +766,48 @@ int sfd_synchronize(sfd_ctx* ctx)
     SFD_TRY(check_ctx(ctx));
     SFD_HIP(hipStreamSynchronize(ctx->stream));
     return SFD_OK;
+}
+
+int sfd_split_step_noalloc(sfd_ctx* ctx, void* psi, const void* V, double coeffD, double beta, double Omega,
+                           double dt, int64_t nsteps, int order, int mode)
+{
+    SFD_TRY(check_ctx(ctx));
+    if (!psi) return fail(SFD_ERR_NULL, "sfd_split_step_noalloc: psi is null");
+    if (order != 1 && order != 2) return fail(SFD_ERR_ARG, "order must be 1 or 2 (got %d)", order);
+    if (mode != SFD_REAL_TIME && mode != SFD_IMAG_TIME) return fail(SFD_ERR_ARG, "mode must be 0 or 1");
+    if (nsteps < 0) return fail(SFD_ERR_ARG, "nsteps must be >= 0");
+    if (!std::isfinite(coeffD) || !std::isfinite(beta) || !std::isfinite(Omega) || !std::isfinite(dt))
+        return fail(SFD_ERR_ARG, "non-finite scalar argument");
+    unsigned char* base = static_cast<unsigned char*>(scratch_ptr(ctx->device));
+    if (!base) return fail(SFD_ERR_HIP, "sfd_split_step_noalloc: no scratch array on device %d", ctx->device);
+    const PrecOps& ops = ops_for(ctx->prec);
+    // layout: [scale and sums: 256 B][norm partials][phase table of each axis]
+    size_t need = 256;
+    long long nparts = 0;
+    for (int ax = 0; ax < ctx->ndim; ++ax) nparts = std::max(nparts, sweep_nblocks(ctx, ax));
+    const size_t part_off = need;
+    if (mode == SFD_IMAG_TIME) need += align256((size_t)nparts * sizeof(double));
+    Workspace ws{};
+    size_t tab_off[3] = {0, 0, 0};
+    for (int ax = 0; ax < ctx->ndim; ++ax) {
+        ws.rows[ax] = phase_rows(ctx, ax, Omega);
+        tab_off[ax] = need;
+        need += align256((size_t)ws.rows[ax] * ctx->n[ax] * ops.table_entry_bytes);
+    }
+    if (need > kScratchBytes)
+        return fail(SFD_ERR_ARG,
+                    "sfd_split_step_noalloc: this grid needs %zu bytes of scratch (phase tables and partial sums), "
+                    "more than the %zu of the library's static array; use sfd_split_step",
+                    need, kScratchBytes);
+    ws.scal = reinterpret_cast<double*>(base);
+    ws.partial = reinterpret_cast<double*>(base + part_off);
+    const double tau = order == 2 ? 0.5 * dt : dt;
+    const int imag = mode == SFD_IMAG_TIME;
+    for (int ax = 0; ax < ctx->ndim; ++ax) {
+        ws.phase[ax] = base + tab_off[ax];
+        SFD_TRY(ops.build_phase_into(ctx, ctx->stream, ws.phase[ax], ax, tau, coeffD, Omega, imag, ws.rows[ax]));
+    }
+    return ops.split(ctx, psi, V, coeffD, beta, Omega, dt, nsteps, order, mode, nullptr, true, &ws);
 }
 
 int sfd_ctx_time_sweeps(sfd_ctx* ctx, int enable)

@@ -35,7 +35,7 @@ EXPORTS = (
     "sfd_upload", "sfd_download", "sfd_synchronize", "sfd_last_error", "sfd_version",
     "sfd_autotune", "sfd_ctx_get_variant", "sfd_ctx_set_variant",
     "sfd_batch_create", "sfd_batch_destroy", "sfd_batch_imag_run", "sfd_batch_autotune", "sfd_adjoint",
-    "sfd_project", "sfd_inner", "sfd_ctx_time_sweeps", "sfd_ctx_sweep_times",
+    "sfd_project", "sfd_inner", "sfd_ctx_time_sweeps", "sfd_ctx_sweep_times", "sfd_split_step_noalloc",
 )
 
 
@@ -81,6 +81,7 @@ def load():
         "sfd_ctx_destroy": ([vp], c_int),
         "sfd_ctx_set_stream": ([vp, vp], c_int),
         "sfd_split_step": ([vp, vp, vp, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_int, c_int], c_int),
+        "sfd_split_step_noalloc": ([vp, vp, vp, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_int, c_int], c_int),
         "sfd_lap_rot": ([vp, vp, c_dbl, c_dbl, c_dbl, c_int], c_int),
         "sfd_lap_rot_axis": ([vp, vp, c_int, c_dbl, c_dbl, c_dbl, c_int], c_int),
         "sfd_solve_nl": ([vp, vp, vp, c_dbl, c_dbl, c_int], c_int),
