@@ -169,32 +169,62 @@ int sfd_ctx_sweep_times(sfd_ctx* ctx, double* ms_sum, int64_t* count, int reset)
 
 /* Batched imaginary-time ground states (no single counterpart in the
  * reference, which solves one field at a time with NumModelBackwardEuler,
- * backward-euler.jl:72-90): ninst independent 2D problems of possibly
- * different extents n[2*i], n[2*i+1] (powers of two, 2..1024) on boxes
- * box[4*i..4*i+3] = xmin, xmax, ymin, ymax. */
+ * backward-euler.jl:72-90): ninst independent problems of possibly
+ * different extents.  sfd_batch_create takes 2D instances, n[2*i],
+ * n[2*i+1] on boxes box[4*i..4*i+3] = xmin, xmax, ymin, ymax;
+ * sfd_batch_create_nd takes ndim[i] = 2 or 3, n[3*i..3*i+2] and
+ * box[6*i..6*i+5] (unused entries ignored).  Extents are powers of two in
+ * [2, 1024]; at most 32 distinct (ndim, extents) shapes. */
 typedef struct sfd_batch sfd_batch;
 int sfd_batch_create(sfd_batch** out, int ninst, const int64_t* n, const double* box, int precision,
                      void* stream);
+int sfd_batch_create_nd(sfd_batch** out, int ninst, const int* ndim, const int64_t* n, const double* box,
+                        int precision, void* stream);
 int sfd_batch_destroy(sfd_batch* b);
 
+#define SFD_BATCH_GRAPH 1    /* replay chunks of check_every steps as hipGraphs */
+#define SFD_BATCH_ESCALATE 2 /* FP32 storage: finish each instance in FP64 (below) */
+
 /* Advance every field psi[i] (V[i] may be NULL; V itself may be NULL) with
- * the imaginary-time Strang split (exp(coeffD xi^2 dt/2) sweeps, then
+ * the imaginary-time Strang split of sfd_split_step(mode = SFD_IMAG_TIME):
+ * sweeps exp((coeffD xi^2 + rotation terms of Omega[i]) dt/2) per axis, then
  * exp(-(V + beta[i]|psi|^2) dt), renormalised to norm 1 after each step in
- * FP64), until ||psi_{n+1} - psi_n|| < tol (checked every check_every steps)
- * or max_steps.  Fields of equal extents share each launch; a chunk of
- * check_every steps is captured once in a hipGraph and replayed when
- * use_graph != 0.  On return every field is normalised; steps_out[i] and
- * dpsi_out[i] (optional) receive the steps taken and the last ||Delta psi||.
- * Synchronises. */
+ * FP64, until ||psi_{n+1} - psi_n|| < tol (checked every check_every steps)
+ * or max_steps.  Omega may be NULL (no rotation).  Fields of equal shape
+ * share each launch; with SFD_BATCH_GRAPH a chunk of check_every steps is
+ * captured once per set of unfinished shape groups and replayed.
+ * With SFD_BATCH_ESCALATE (FP32 or F32X batches): the rounding of FP32
+ * states keeps ||Delta psi|| near 1e-7, so each instance runs in its FP32
+ * storage until its increment is below 1e-6, or below 1e-4 while shrinking by
+ * less than 0.8 per check, and then continues from that state in FP64 (FP64
+ * field, FP64 snapshot, FP64 ||Delta psi||) until tol; with tol >= 1e-6 an
+ * instance may also finish in FP32.  Without rotation (a real Hamiltonian,
+ * whose ground state is real up to a global phase) the state taken to FP64
+ * is put back on the dominant phase phi0 of its initial state,
+ * psi <- e^{i phi0} Re(e^{-i phi} psi), phi = arg(sum psi^2)/2 of the state:
+ * the exact flow keeps that phase, and the FP32 rounding noise out of it
+ * would decay only at the slow rate of phase gradients.  On return every field is normalised
+ * (an escalated one rounded once from its FP64 state); psi64_out[i]
+ * (optional, complex FP64 device buffers of the instance's size, entries
+ * may be NULL) receives the final state in FP64; steps_out[i], dpsi_out[i]
+ * and level_out[i] (optional) receive the steps taken, the last
+ * ||Delta psi|| and 1 if the instance finished in FP64.  Synchronises. */
+int sfd_batch_imag_solve(sfd_batch* b, void* const* psi, const void* const* V, const double* beta,
+                         const double* Omega, double coeffD, double dt, int64_t max_steps, double tol,
+                         int check_every, int flags, int64_t* steps_out, double* dpsi_out,
+                         void* const* psi64_out, int* level_out);
+
+/* sfd_batch_imag_solve with Omega = NULL, flags = use_graph ? SFD_BATCH_GRAPH : 0
+ * and no FP64 outputs (the interface of the first ABI version). */
 int sfd_batch_imag_run(sfd_batch* b, void* const* psi, const void* const* V, const double* beta,
                        double coeffD, double dt, int64_t max_steps, double tol, int check_every,
                        int use_graph, int64_t* steps_out, double* dpsi_out);
 
-/* Autotuner of a batch: for every size group and axis, times each compiled
+/* Autotuner of a batch: for every shape group and axis, times each compiled
  * launch variant of the batched sweep (identity round trips on the batch's
  * own scratch, the fields are not touched) and keeps the fastest, like
  * sfd_autotune for one context.  times_ms (may be NULL) receives at least
- * 16*ninst floats: [16 g + 8 axis + variant] = ms per launch, -1 where there
+ * 24*ninst floats: [24 g + 8 axis + variant] = ms per launch, -1 where there
  * is none, g in the batch's internal group order.  No reference counterpart
  * (FFTW plan flags, Plans.jl:847-848, play this role there). */
 int sfd_batch_autotune(sfd_batch* b, float* times_ms);

@@ -281,40 +281,51 @@ void release(sfd_ctx* c)
     if (c->scal) (void)hipFree(c->scal);
 }
 
-// Line geometry of instance `in` along `ax` under its group's variant.
-void batch_geom(SweepArgs& a, const sfd_batch::Inst& in, const sfd_batch::Group& G, int ax)
+// Line geometry of instance `in` along `ax` under its group's variant at
+// `level` (2D and 3D; the strided axes take x-columns in tiles of NL).
+void batch_geom(SweepArgs& a, const sfd_batch::Inst& in, const sfd_batch::Group& G, int level, int ax)
 {
+    const long long nx = in.n[0], ny = in.n[1], nz = in.ndim == 3 ? in.n[2] : 1;
     if (ax == 0) {
         a.S = 1;
-        a.nlines = in.n[1];
+        a.nlines = ny * nz;
     } else {
-        a.S = in.n[0];
-        a.OS = (long long)in.n[0] * in.n[1];
-        a.ncols = in.n[0];
-        a.ntx = (int)G.ntiles[1];
+        a.ncols = (int)nx;
+        if (ax == 1) { a.S = nx; a.OS = nx * ny; }
+        else         { a.S = nx * ny; a.OS = nx; }
+        a.ntx = G.ntx[level][ax];
     }
-    a.ntiles = G.ntiles[ax];
+    a.ntiles = G.ntiles[level][ax];
 }
 
-void batch_drop_graph(sfd_batch* b)
+void batch_drop_graphs(sfd_batch* b)
 {
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->gexec = nullptr;
-    b->graph = nullptr;
-    b->graph_k = 0;
+    for (auto& cg : b->graphs) {
+        if (cg.exec) (void)hipGraphExecDestroy(cg.exec);
+        if (cg.graph) (void)hipGraphDestroy(cg.graph);
+    }
+    b->graphs.clear();
 }
 
 void batch_release(sfd_batch* b)
 {
-    batch_drop_graph(b);
+    batch_drop_graphs(b);
     for (auto& G : b->groups)
-        for (int ax = 0; ax < 2; ++ax)
-            if (G.tw[ax]) (void)hipFree(G.tw[ax]);
-    for (void* p : b->phase)
-        if (p) (void)hipFree(p);
-    void* ptrs[] = {b->d_inst, b->d_scale, b->d_snap_scale, b->d_dpsi, b->d_done, b->d_steps,
-                    b->d_partial, b->d_dpart, b->d_snap, b->d_desc};
+        for (int L = 0; L < sfd_batch::kLevels; ++L)
+            for (int ax = 0; ax < 3; ++ax)
+                if (G.tw[L][ax]) (void)hipFree(G.tw[L][ax]);
+    for (auto& lv : b->lv) {
+        for (void* p : lv.phase)
+            if (p) (void)hipFree(p);
+        void* ptrs[] = {lv.d_inst, lv.d_desc, lv.d_done, lv.d_partial, lv.d_snap, lv.d_field, lv.d_V};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+    for (auto& in : b->inst)
+        for (double* g : in.geo)
+            if (g) (void)hipFree(g);
+    void* ptrs[] = {b->d_scale, b->d_snap_scale, b->d_dpsi, b->d_dprev, b->d_esc,   b->d_steps,
+                    b->d_dpart, b->d_out64,     b->d_real, b->d_sq,   b->d_phi0};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->ev) (void)hipEventDestroy(b->ev);
@@ -323,6 +334,289 @@ void batch_release(sfd_batch* b)
     for (hipStream_t st : b->gstream) (void)hipStreamDestroy(st);
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
+
+namespace {
+
+// Groups of equal (ndim, extents) in group-major internal order, the FP64
+// geometry of every instance, the shared per-instance arrays, one stream per
+// group, and level 0.
+int batch_setup(sfd_batch* b)
+{
+    std::vector<sfd_batch::Inst> sorted;
+    for (auto& in : b->inst) {
+        int gidx = -1;
+        for (size_t g = 0; g < b->groups.size(); ++g) {
+            const auto& G = b->groups[g];
+            if (G.ndim == in.ndim && G.n[0] == in.n[0] && G.n[1] == in.n[1] && G.n[2] == in.n[2]) gidx = (int)g;
+        }
+        if (gidx < 0) {
+            sfd_batch::Group G;
+            G.ndim = in.ndim;
+            for (int a = 0; a < 3; ++a) G.n[a] = in.n[a];
+            b->groups.push_back(G);
+            gidx = (int)b->groups.size() - 1;
+        }
+        in.group = gidx;
+    }
+    if (b->groups.size() > 32) return fail(SFD_ERR_ARG, "batch: at most 32 distinct grid shapes (got %zu)",
+                                           b->groups.size());
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        b->groups[g].first = (int)sorted.size();
+        for (auto& in : b->inst)
+            if (in.group == (int)g) sorted.push_back(in);
+        b->groups[g].count = (int)sorted.size() - b->groups[g].first;
+    }
+    b->inst = sorted;
+    b->sites = 0;
+    for (auto& in : b->inst) {
+        in.soff = b->sites;
+        b->sites += in.nsite;
+        for (int ax = 0; ax < in.ndim; ++ax) {
+            const int n = in.n[ax];
+            std::vector<double> t(2 * (size_t)n);
+            for (int k = 0; k < n; ++k) {
+                t[k] = fft_freq(n, in.d[ax], k);
+                t[n + k] = lin_coord(in.lo[ax], in.hi[ax], n, k);
+            }
+            SFD_HIP(hipMalloc(&in.geo[ax], t.size() * sizeof(double)));
+            SFD_TRY(upload_sync(in.geo[ax], t.data(), t.size() * sizeof(double)));
+        }
+    }
+    const int ni = b->ninst;
+    SFD_HIP(hipMalloc(&b->d_scale, ni * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_snap_scale, ni * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_dpsi, ni * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_dprev, ni * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_esc, ni * sizeof(int)));
+    SFD_HIP(hipMalloc(&b->d_real, ni * sizeof(int)));
+    SFD_HIP(hipMalloc(&b->d_sq, (size_t)ni * 2 * sfd_batch::kDiffBlocks * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_phi0, ni * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_steps, ni * sizeof(long long)));
+    SFD_HIP(hipMalloc(&b->d_dpart, (size_t)ni * sfd_batch::kDiffBlocks * sizeof(double)));
+    SFD_HIP(hipMalloc(&b->d_out64, ni * sizeof(void*)));
+    SFD_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        SFD_HIP(hipStreamCreate(&st));
+        b->gstream.push_back(st);
+        SFD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        b->gdone.push_back(ev);
+    }
+    b->lv[0].prec = b->prec;
+    return ops_for(b->prec).batch_level_setup(b, 0);
+}
+
+// Enqueue k imaginary-time Strang steps of every (group, level) set in
+// `mask` (bit 2 g + level); the last step is a convergence check, after
+// which level-0 instances at their FP32 floor move to FP64 when
+// `escalate`.  Each size group runs on its own stream (forked from and
+// joined back to the batch stream), so the groups' small launches overlap.
+int batch_enqueue(sfd_batch* b, unsigned long long mask, int k, double tol, bool escalate)
+{
+    SFD_HIP(hipEventRecord(b->fork, b->stream));
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        const bool act[2] = {((mask >> (2 * g)) & 1ull) != 0, ((mask >> (2 * g + 1)) & 1ull) != 0};
+        if (!act[0] && !act[1]) continue;
+        hipStream_t st = b->gstream[g];
+        SFD_HIP(hipStreamWaitEvent(st, b->fork, 0));
+        for (int s = 0; s < k; ++s) {
+            const bool chk = s == k - 1;
+            for (int L = 0; L < sfd_batch::kLevels; ++L)
+                if (act[L])
+                    SFD_TRY(ops_for(b->lv[L].prec).batch_level_step(b, L, (int)g, st, chk, tol, L == 0 && escalate));
+            if (chk && escalate && act[0]) SFD_TRY(ops_for(b->prec).batch_escalate(b, (int)g, st));
+        }
+        SFD_HIP(hipEventRecord(b->gdone[g], st));
+        SFD_HIP(hipStreamWaitEvent(b->stream, b->gdone[g], 0));
+    }
+    return SFD_OK;
+}
+
+// The step of each group for this solve: x, [y N y], x in 2D when no
+// instance of the group rotates (the folded Strang step), x, y + N, x, y in
+// 2D with rotation (K_x and K_y do not commute), x, y, [z N z], x, y in 3D.
+// The first sweep applies the pending renormalisation, the last one sums
+// the norm.
+void batch_kinds(sfd_batch* b, const double* Omega)
+{
+    using K = sfd_batch::Kind;
+    for (auto& G : b->groups) {
+        bool rot = false;
+        for (int i = G.first; i < G.first + G.count; ++i)
+            rot = rot || (Omega && Omega[b->inst[i].user] != 0.0);
+        if (G.ndim == 3)
+            G.kinds = {K{0, kFlagScale, false}, K{1, 0, false}, K{2, kFlagNL, true}, K{0, 0, false},
+                       K{1, kFlagNorm, false}};
+        else if (!rot)
+            G.kinds = {K{0, kFlagScale, false}, K{1, kFlagNL, true}, K{0, kFlagNorm, false}};
+        else
+            G.kinds = {K{0, kFlagScale, false}, K{1, kFlagNL, false}, K{0, 0, false}, K{1, kFlagNorm, false}};
+    }
+}
+
+int batch_solve(sfd_batch* b, void* const* psi, const void* const* V, const double* beta, const double* Omega,
+                double cD, double dt, int64_t max_steps, double tol, int check_every, int flags, int64_t* steps_out,
+                double* dpsi_out, void* const* psi64_out, int* level_out)
+{
+    const int ni = b->ninst;
+    const bool escalate = (flags & SFD_BATCH_ESCALATE) && b->prec != SFD_F64;
+    const bool use_graph = (flags & SFD_BATCH_GRAPH) != 0;
+    for (int i = 0; i < ni; ++i)
+        if (!psi[i]) return fail(SFD_ERR_NULL, "batch: field %d is null", i);
+    if (escalate && !b->lv[1].prec) {
+        b->lv[1].prec = SFD_F64;
+        SFD_TRY(ops_for(SFD_F64).batch_level_setup(b, 1));
+    }
+    batch_kinds(b, Omega);
+    batch_drop_graphs(b);
+    // order after the caller's pending work
+    SFD_HIP(hipEventRecord(b->ev, b->user_stream));
+    SFD_HIP(hipStreamWaitEvent(b->stream, b->ev, 0));
+    SFD_TRY(ops_for(b->prec).batch_level_bind(b, 0, psi, V, beta, Omega, cD, dt));
+    if (escalate) SFD_TRY(ops_for(SFD_F64).batch_level_bind(b, 1, psi, V, beta, Omega, cD, dt));
+    std::vector<void*> out64(ni, nullptr);
+    std::vector<int> real(ni, 1);
+    for (int i = 0; i < ni; ++i) {
+        if (psi64_out) out64[i] = psi64_out[b->inst[i].user];
+        if (Omega && Omega[b->inst[i].user] != 0.0) real[i] = 0;
+    }
+    SFD_HIP(hipMemcpyAsync(b->d_out64, out64.data(), ni * sizeof(void*), hipMemcpyHostToDevice, b->stream));
+    SFD_HIP(hipMemcpyAsync(b->d_real, real.data(), ni * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(k_batch_reset, dim3((ni + 255) / 256), dim3(256), 0, b->stream, b->d_scale, b->lv[0].d_done,
+                       escalate ? b->lv[1].d_done : (int*)nullptr, b->d_steps, b->d_dpsi, b->d_dprev, b->d_esc, ni);
+    SFD_HIP(hipGetLastError());
+    if (escalate) SFD_TRY(ops_for(b->prec).batch_phase0(b));
+    std::vector<int> done0(ni, 0), done1(ni, 1);
+    unsigned long long mask = 0;
+    for (size_t g = 0; g < b->groups.size(); ++g) mask |= 1ull << (2 * g);
+    int64_t left = max_steps;
+    while (left > 0 && mask) {
+        const int k = (int)(left < check_every ? left : check_every);
+        if (use_graph && k == check_every) {
+            sfd_batch::CachedGraph* cg = nullptr;
+            for (auto& c : b->graphs)
+                if (c.mask == mask && c.k == k) cg = &c;
+            if (!cg) {
+                if (b->graphs.size() >= 8) batch_drop_graphs(b);
+                SFD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                int rc = batch_enqueue(b, mask, k, tol, escalate);
+                hipGraph_t graph = nullptr;
+                hipError_t e = hipStreamEndCapture(b->stream, &graph);
+                if (rc != SFD_OK) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return rc;
+                }
+                if (e != hipSuccess) return fail(SFD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+                hipGraphExec_t exec = nullptr;
+                e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                if (e != hipSuccess) {
+                    (void)hipGraphDestroy(graph);
+                    return fail(SFD_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+                }
+                b->graphs.push_back({mask, k, graph, exec});
+                cg = &b->graphs.back();
+            }
+            SFD_HIP(hipGraphLaunch(cg->exec, b->stream));
+        } else {
+            SFD_TRY(batch_enqueue(b, mask, k, tol, escalate));
+        }
+        left -= k;
+        SFD_HIP(hipMemcpyAsync(done0.data(), b->lv[0].d_done, ni * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        if (escalate)
+            SFD_HIP(hipMemcpyAsync(done1.data(), b->lv[1].d_done, ni * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        SFD_HIP(hipStreamSynchronize(b->stream));
+        mask = 0;
+        for (size_t g = 0; g < b->groups.size(); ++g) {
+            const auto& G = b->groups[g];
+            for (int i = G.first; i < G.first + G.count; ++i) {
+                if (!done0[i]) mask |= 1ull << (2 * g);
+                if (escalate && !done1[i]) mask |= 1ull << (2 * g + 1);
+            }
+        }
+    }
+    // leave every field normalised, and in FP64 where asked
+    SFD_TRY(ops_for(b->prec).batch_finish(b));
+    std::vector<long long> st(ni);
+    std::vector<double> dp(ni);
+    std::vector<int> esc(ni);
+    SFD_HIP(hipMemcpyAsync(st.data(), b->d_steps, ni * sizeof(long long), hipMemcpyDeviceToHost, b->stream));
+    SFD_HIP(hipMemcpyAsync(dp.data(), b->d_dpsi, ni * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    SFD_HIP(hipMemcpyAsync(esc.data(), b->d_esc, ni * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    SFD_HIP(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < ni; ++i) {
+        const int u = b->inst[i].user;
+        if (steps_out) steps_out[u] = st[i];
+        if (dpsi_out) dpsi_out[u] = dp[i];
+        if (level_out) level_out[u] = esc[i] == 2 ? 1 : 0;
+    }
+    return SFD_OK;
+}
+
+// Instance records of sfd_batch_create_nd (bounds checked, extents checked).
+int batch_create(sfd_batch** out, int ninst, const int* ndim, const int64_t* n, const double* box, int precision,
+                 void* stream)
+{
+    g_last_error.clear();
+    if (!out || !n || !box) return fail(SFD_ERR_NULL, "sfd_batch_create: null argument");
+    *out = nullptr;
+    if (ninst < 1) return fail(SFD_ERR_ARG, "sfd_batch_create: ninst must be >= 1");
+    if (precision != SFD_F32 && precision != SFD_F64 && precision != SFD_F32X)
+        return fail(SFD_ERR_ARG, "precision must be SFD_F32, SFD_F64 or SFD_F32X (got %d)", precision);
+    std::vector<sfd_batch::Inst> inst;
+    for (int i = 0; i < ninst; ++i) {
+        sfd_batch::Inst in{};
+        in.ndim = ndim ? ndim[i] : 2;
+        if (in.ndim != 2 && in.ndim != 3) return fail(SFD_ERR_SHAPE, "instance %d: ndim must be 2 or 3", i);
+        in.nsite = 1;
+        for (int a = 0; a < 3; ++a) {
+            in.n[a] = 1;
+            in.lo[a] = 0.0;
+            in.hi[a] = 1.0;
+            in.d[a] = 1.0;
+            in.geo[a] = nullptr;
+        }
+        for (int a = 0; a < in.ndim; ++a) {
+            const int64_t m = ndim ? n[3 * i + a] : n[2 * i + a];
+            if (m < 2 || m > 1024 || (m & (m - 1)))
+                return fail(SFD_ERR_SHAPE, "instance %d axis %d: n = %lld is not a power of two in [2, 1024]", i, a,
+                            (long long)m);
+            in.n[a] = (int)m;
+            in.lo[a] = ndim ? box[6 * i + 2 * a] : box[4 * i + 2 * a];
+            in.hi[a] = ndim ? box[6 * i + 2 * a + 1] : box[4 * i + 2 * a + 1];
+            if (!(in.hi[a] > in.lo[a]) || !std::isfinite(in.lo[a]) || !std::isfinite(in.hi[a]))
+                return fail(SFD_ERR_ARG, "instance %d axis %d: bounds must satisfy min < max", i, a);
+            in.d[a] = (in.hi[a] - in.lo[a]) / (double)m;
+            in.nsite *= m;
+        }
+        in.group = -1;
+        in.user = i;
+        inst.push_back(in);
+    }
+    auto* b = new sfd_batch();
+    b->ninst = ninst;
+    b->prec = precision;
+    b->user_stream = static_cast<hipStream_t>(stream);
+    b->inst = inst;
+    hipError_t e = hipGetDevice(&b->device);
+    if (e == hipSuccess) e = hipStreamCreate(&b->stream);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        batch_release(b);
+        delete b;
+        return fail(SFD_ERR_HIP, "sfd_batch_create: %s", hipGetErrorString(e));
+    }
+    const int rc = batch_setup(b);
+    if (rc != SFD_OK) {
+        batch_release(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return SFD_OK;
+}
+
+}  // namespace
 
 // Sweep timer: pair k uses events 2k (before the launch) and 2k + 1 (after
 // it), both on the context's stream; a full pool is summed before reuse.
@@ -547,56 +841,14 @@ int sfd_fft_axis(sfd_ctx* ctx, const void* in, void* out, int axis, int directio
 
 int sfd_batch_create(sfd_batch** out, int ninst, const int64_t* n, const double* box, int precision, void* stream)
 {
-    g_last_error.clear();
-    if (!out || !n || !box) return fail(SFD_ERR_NULL, "sfd_batch_create: null argument");
-    *out = nullptr;
-    if (ninst < 1) return fail(SFD_ERR_ARG, "sfd_batch_create: ninst must be >= 1");
-    if (precision != SFD_F32 && precision != SFD_F64 && precision != SFD_F32X)
-        return fail(SFD_ERR_ARG, "precision must be SFD_F32, SFD_F64 or SFD_F32X (got %d)", precision);
-    auto* b = new sfd_batch();
-    b->ninst = ninst;
-    b->prec = precision;
-    b->user_stream = static_cast<hipStream_t>(stream);
-    for (int i = 0; i < ninst; ++i) {
-        sfd_batch::Inst in;
-        in.nsite = 1;
-        for (int a = 0; a < 2; ++a) {
-            const int64_t m = n[2 * i + a];
-            if (m < 2 || m > 1024 || (m & (m - 1))) {
-                delete b;
-                return fail(SFD_ERR_SHAPE, "instance %d axis %d: n = %lld is not a power of two in [2, 1024]", i, a,
-                            (long long)m);
-            }
-            in.n[a] = (int)m;
-            in.lo[a] = box[4 * i + 2 * a];
-            in.hi[a] = box[4 * i + 2 * a + 1];
-            if (!(in.hi[a] > in.lo[a])) {
-                delete b;
-                return fail(SFD_ERR_ARG, "instance %d axis %d: bounds must satisfy min < max", i, a);
-            }
-            in.d[a] = (in.hi[a] - in.lo[a]) / (double)m;
-            in.nsite *= m;
-        }
-        in.group = -1;
-        in.user = i;
-        b->inst.push_back(in);
-    }
-    hipError_t e = hipGetDevice(&b->device);
-    if (e == hipSuccess) e = hipStreamCreate(&b->stream);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
-    if (e != hipSuccess) {
-        batch_release(b);
-        delete b;
-        return fail(SFD_ERR_HIP, "sfd_batch_create: %s", hipGetErrorString(e));
-    }
-    int rc = ops_for(b->prec).batch_setup(b);
-    if (rc != SFD_OK) {
-        batch_release(b);
-        delete b;
-        return rc;
-    }
-    *out = b;
-    return SFD_OK;
+    return batch_create(out, ninst, nullptr, n, box, precision, stream);
+}
+
+int sfd_batch_create_nd(sfd_batch** out, int ninst, const int* ndim, const int64_t* n, const double* box,
+                        int precision, void* stream)
+{
+    if (!ndim) return fail(SFD_ERR_NULL, "sfd_batch_create_nd: ndim is null");
+    return batch_create(out, ninst, ndim, n, box, precision, stream);
 }
 
 int sfd_batch_autotune(sfd_batch* b, float* times_ms)
@@ -617,18 +869,32 @@ int sfd_batch_destroy(sfd_batch* b)
     return SFD_OK;
 }
 
+int sfd_batch_imag_solve(sfd_batch* b, void* const* psi, const void* const* V, const double* beta,
+                         const double* Omega, double coeffD, double dt, int64_t max_steps, double tol,
+                         int check_every, int flags, int64_t* steps_out, double* dpsi_out, void* const* psi64_out,
+                         int* level_out)
+{
+    g_last_error.clear();
+    if (!b) return fail(SFD_ERR_NULL, "null batch");
+    if (!psi || !beta) return fail(SFD_ERR_NULL, "sfd_batch_imag_solve: null argument");
+    if (max_steps < 0 || check_every < 1) return fail(SFD_ERR_ARG, "max_steps >= 0 and check_every >= 1 required");
+    if (flags & ~(SFD_BATCH_GRAPH | SFD_BATCH_ESCALATE)) return fail(SFD_ERR_ARG, "unknown flags 0x%x", flags);
+    if (!std::isfinite(coeffD) || !std::isfinite(dt) || !std::isfinite(tol))
+        return fail(SFD_ERR_ARG, "non-finite scalar argument");
+    for (int i = 0; i < b->ninst; ++i)
+        if (!std::isfinite(beta[i]) || (Omega && !std::isfinite(Omega[i])))
+            return fail(SFD_ERR_ARG, "instance %d: non-finite beta or Omega", i);
+    SFD_HIP(hipSetDevice(b->device));
+    return batch_solve(b, psi, V, beta, Omega, coeffD, dt, max_steps, tol, check_every, flags, steps_out, dpsi_out,
+                       psi64_out, level_out);
+}
+
 int sfd_batch_imag_run(sfd_batch* b, void* const* psi, const void* const* V, const double* beta, double coeffD,
                        double dt, int64_t max_steps, double tol, int check_every, int use_graph, int64_t* steps_out,
                        double* dpsi_out)
 {
-    if (!b) return fail(SFD_ERR_NULL, "null batch");
-    if (!psi || !beta) return fail(SFD_ERR_NULL, "sfd_batch_imag_run: null argument");
-    if (max_steps < 0 || check_every < 1) return fail(SFD_ERR_ARG, "max_steps >= 0 and check_every >= 1 required");
-    if (!std::isfinite(coeffD) || !std::isfinite(dt) || !std::isfinite(tol))
-        return fail(SFD_ERR_ARG, "non-finite scalar argument");
-    SFD_HIP(hipSetDevice(b->device));
-    return ops_for(b->prec).batch_run(b, psi, V, beta, coeffD, dt, max_steps, tol, check_every, use_graph,
-                       steps_out, dpsi_out);
+    return sfd_batch_imag_solve(b, psi, V, beta, nullptr, coeffD, dt, max_steps, tol, check_every,
+                                use_graph ? SFD_BATCH_GRAPH : 0, steps_out, dpsi_out, nullptr, nullptr);
 }
 
 int sfd_adjoint(sfd_ctx* ctx, const void* psi0, const void* V, const void* target, double coeffD, double beta,

@@ -735,242 +735,315 @@ int init_tables(sfd_ctx* c)
     return SFD_OK;
 }
 
-// Launch variant `idx` for one axis of a size group: its twiddles (rebuilt
-// when the line split changes) and tile count.
+// ------------------------------------------------------------ batch levels
+// A batch advances its instances at level 0 (the batch's precision) and, for
+// FP32 storage with escalation, at level 1 (FP64) once they reach their
+// FP32 floor.  The routines below run one level with the kernels of
+// precision P; the chunk loop, graphs and escalation policy are in sfd.hip.
+
+// FP32 potential widened to the FP64 level's copy.
+template <typename S>
+__global__ void k_widen_real(const S* __restrict__ src, double* __restrict__ dst, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = (double)src[i];
+}
+
+// Launch variant `idx` for one axis of a size group at one level: its
+// twiddles (rebuilt when the line split changes) and tile counts.
 template <typename P>
-int batch_group_variant(sfd_batch* b, sfd_batch::Group& G, int ax, int idx)
+int batch_level_variant(sfd_batch* b, int L, int g, int ax, int idx)
 {
     using R = typename P::R;
+    auto& G = b->groups[g];
     VariantInfo vi;
     if (!variant_info<P>(G.n[ax], ax == 0, idx, &vi))
-        return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", idx, G.n[ax]);
-    if (!G.tw[ax] || G.vi[ax].n1 != vi.n1 || G.vi[ax].n2 != vi.n2 || G.vi[ax].h != vi.h) {
+        return fail(SFD_ERR_SHAPE, "batch: no sweep variant %d for line length %d", idx, G.n[ax]);
+    if (!G.tw[L][ax] || G.vi[L][ax].n1 != vi.n1 || G.vi[L][ax].n2 != vi.n2 || G.vi[L][ax].h != vi.h) {
         std::vector<cx<R>> w;
         fill_twiddles<R>(w, G.n[ax], vi.n1, vi.n2, vi.h);
-        if (G.tw[ax]) {
+        if (G.tw[L][ax]) {
             SFD_HIP(hipStreamSynchronize(b->stream));
-            SFD_HIP(hipFree(G.tw[ax]));
-            G.tw[ax] = nullptr;
+            SFD_HIP(hipFree(G.tw[L][ax]));
+            G.tw[L][ax] = nullptr;
         }
-        SFD_HIP(hipMalloc(&G.tw[ax], w.size() * sizeof(cx<R>)));
-        SFD_TRY(upload_sync(G.tw[ax], w.data(), w.size() * sizeof(cx<R>)));
+        SFD_HIP(hipMalloc(&G.tw[L][ax], w.size() * sizeof(cx<R>)));
+        SFD_TRY(upload_sync(G.tw[L][ax], w.data(), w.size() * sizeof(cx<R>)));
     }
-    G.vi[ax] = vi;
-    G.var[ax] = idx;
-    const int NL = vi.ts * vi.g;
-    G.ntiles[ax] = ax == 0 ? (G.n[1] + NL - 1) / NL : (G.n[0] + NL - 1) / NL;
+    G.vi[L][ax] = vi;
+    G.var[L][ax] = idx;
+    const long long NL = (long long)vi.ts * vi.g;
+    const long long nx = G.n[0], ny = G.n[1], nz = G.ndim == 3 ? G.n[2] : 1;
+    if (ax == 0) {
+        G.ntx[L][ax] = 0;
+        G.ntiles[L][ax] = (ny * nz + NL - 1) / NL;
+    } else {
+        G.ntx[L][ax] = (int)((nx + NL - 1) / NL);
+        G.ntiles[L][ax] = (long long)G.ntx[L][ax] * (ax == 1 ? nz : ny);
+    }
     return SFD_OK;
 }
 
+// Tiles of the most-tiled variant of axis ax for a group: the room for the
+// norm partials of a sweep on that axis whatever the autotuner picks.
 template <typename P>
-int batch_setup(sfd_batch* b)
+long long batch_most_tiles(const sfd_batch::Group& G, int ax)
+{
+    long long most = 0;
+    const long long nx = G.n[0], ny = G.n[1], nz = G.ndim == 3 ? G.n[2] : 1;
+    for (int v = 0; v < n_variants<P>(G.n[ax], ax == 0); ++v) {
+        VariantInfo vi;
+        if (!variant_info<P>(G.n[ax], ax == 0, v, &vi)) continue;
+        const long long NL = (long long)vi.ts * vi.g;
+        const long long t = ax == 0 ? (ny * nz + NL - 1) / NL : ((nx + NL - 1) / NL) * (ax == 1 ? nz : ny);
+        most = std::max(most, t);
+    }
+    return most;
+}
+
+// Variants, twiddles and device buffers of one level (the norm partials
+// have room for the norm sweep on x or y under any variant).
+template <typename P>
+int batch_level_setup(sfd_batch* b, int L)
 {
     using S = typename P::S;
-    // groups of equal extents, internal order group-major
-    std::vector<sfd_batch::Inst> sorted;
-    for (auto& in : b->inst) {
-        int gidx = -1;
-        for (size_t g = 0; g < b->groups.size(); ++g)
-            if (b->groups[g].n[0] == in.n[0] && b->groups[g].n[1] == in.n[1]) gidx = (int)g;
-        if (gidx < 0) {
-            sfd_batch::Group G;
-            G.n[0] = in.n[0];
-            G.n[1] = in.n[1];
-            b->groups.push_back(G);
-            gidx = (int)b->groups.size() - 1;
-        }
-        in.group = gidx;
-    }
-    for (size_t g = 0; g < b->groups.size(); ++g) {
-        b->groups[g].first = (int)sorted.size();
-        for (auto& in : b->inst)
-            if (in.group == (int)g) sorted.push_back(in);
-        b->groups[g].count = (int)sorted.size() - b->groups[g].first;
-    }
-    b->inst = sorted;
+    auto& lv = b->lv[L];
     const int ni = b->ninst;
-    long long parts = 0, sites = 0;
-    for (auto& G : b->groups) {
-        for (int ax = 0; ax < 2; ++ax) SFD_TRY(batch_group_variant<P>(b, G, ax, 0));
-        // norm partials come from the last x sweep (one per tile of G.n[1]
-        // x lines): room for the variant with the most tiles, so the
-        // autotuner may switch variants
-        long long most = 0;
-        for (int v = 0; v < n_variants<P>(G.n[0], true); ++v) {
-            VariantInfo vi;
-            if (variant_info<P>(G.n[0], true, v, &vi)) {
-                const int NL = vi.ts * vi.g;
-                most = std::max(most, (long long)((G.n[1] + NL - 1) / NL));
-            }
-        }
-        parts += most * G.count;
+    for (size_t g = 0; g < b->groups.size(); ++g)
+        for (int ax = 0; ax < b->groups[g].ndim; ++ax) SFD_TRY(batch_level_variant<P>(b, L, (int)g, ax, 0));
+    lv.poff.assign(ni, 0);
+    long long parts = 0;
+    for (int i = 0; i < ni; ++i) {
+        const auto& G = b->groups[b->inst[i].group];
+        lv.poff[i] = parts;
+        parts += std::max(batch_most_tiles<P>(G, 0), batch_most_tiles<P>(G, 1));
     }
-    for (auto& in : b->inst) sites += in.nsite;
-    SFD_HIP(hipMalloc(&b->d_inst, ni * sizeof(BatchInstDev)));
-    SFD_HIP(hipMalloc(&b->d_scale, ni * sizeof(double)));
-    SFD_HIP(hipMalloc(&b->d_snap_scale, ni * sizeof(double)));
-    SFD_HIP(hipMalloc(&b->d_dpsi, ni * sizeof(double)));
-    SFD_HIP(hipMalloc(&b->d_done, ni * sizeof(int)));
-    SFD_HIP(hipMalloc(&b->d_steps, ni * sizeof(long long)));
-    SFD_HIP(hipMalloc(&b->d_partial, parts * sizeof(double)));
-    SFD_HIP(hipMalloc(&b->d_dpart, (size_t)ni * sfd_batch::kDiffBlocks * sizeof(double)));
-    SFD_HIP(hipMalloc(&b->d_snap, (size_t)sites * sizeof(cx<S>)));
-    SFD_HIP(hipMalloc(&b->d_desc, sfd_batch::kKinds * (size_t)ni * sizeof(SweepArgs)));
-    b->phase.assign(2 * (size_t)ni, nullptr);
-    SFD_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-    for (size_t g = 0; g < b->groups.size(); ++g) {
-        hipStream_t st = nullptr;
-        hipEvent_t ev = nullptr;
-        SFD_HIP(hipStreamCreate(&st));
-        b->gstream.push_back(st);
-        SFD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        b->gdone.push_back(ev);
+    SFD_HIP(hipMalloc(&lv.d_inst, ni * sizeof(BatchInstDev)));
+    SFD_HIP(hipMalloc(&lv.d_desc, (size_t)sfd_batch::kMaxKinds * ni * sizeof(SweepArgs)));
+    SFD_HIP(hipMalloc(&lv.d_done, ni * sizeof(int)));
+    SFD_HIP(hipMalloc(&lv.d_partial, (size_t)parts * sizeof(double)));
+    SFD_HIP(hipMalloc(&lv.d_snap, (size_t)b->sites * sizeof(cx<S>)));
+    if (L == 1) {
+        SFD_HIP(hipMalloc(&lv.d_field, (size_t)b->sites * sizeof(cx<S>)));
+        SFD_HIP(hipMalloc(&lv.d_V, (size_t)b->sites * sizeof(S)));
     }
+    lv.phase.assign(3 * (size_t)ni, nullptr);
+    lv.phase_bytes.assign(3 * (size_t)ni, 0);
     return SFD_OK;
 }
 
-// Descriptors and phase tables of one solve (fields, potentials, beta, dt).
+// Descriptors, phase tables and instance records of one solve at one level.
+// Level 0 works on the caller's fields and potentials; level 1 on its own
+// FP64 fields and FP64 copies of the potentials.
 template <typename P>
-int batch_bind(sfd_batch* b, void* const* psi, const void* const* V, const double* beta, double cD, double dt)
+int batch_level_bind(sfd_batch* b, int L, void* const* psi, const void* const* V, const double* beta,
+                     const double* Omega, double cD, double dt)
 {
     using R = typename P::R;
     using S = typename P::S;
+    auto& lv = b->lv[L];
     const int ni = b->ninst;
     std::vector<BatchInstDev> hi(ni);
-    std::vector<SweepArgs> desc(sfd_batch::kKinds * (size_t)ni);
-    long long poff = 0, soff = 0;
+    std::vector<SweepArgs> desc((size_t)sfd_batch::kMaxKinds * ni);
     const double tau = 0.5 * dt;
     for (int i = 0; i < ni; ++i) {
         const auto& in = b->inst[i];
         const auto& G = b->groups[in.group];
-        if (!psi[in.user]) return fail(SFD_ERR_NULL, "batch: field %d is null", in.user);
-        hi[i].psi = psi[in.user];
-        hi[i].snap = static_cast<char*>(b->d_snap) + soff * sizeof(cx<S>);
-        hi[i].partial = b->d_partial + poff;
-        hi[i].nsite = in.nsite;
-        hi[i].nparts = G.ntiles[0];
-        hi[i].dv = in.d[0] * in.d[1];
-        // imaginary-time multipliers exp(cD xi^2 tau)/n of both axes
-        for (int ax = 0; ax < 2; ++ax) {
-            const int n = in.n[ax];
-            std::vector<cx<R>> h(n);
-            for (int k = 0; k < n; ++k) {
-                const double xi = fft_freq(n, in.d[ax], k);
-                h[k] = decay_entry<R>((cD * (xi * xi)) * tau, 1.0 / (double)n);
+        const double Om = Omega ? Omega[in.user] : 0.0;
+        const void* Vi = V ? V[in.user] : nullptr;
+        void* field = psi[in.user];
+        if (L == 1) {
+            field = static_cast<char*>(lv.d_field) + (size_t)in.soff * sizeof(cx<S>);
+            if (Vi) {
+                double* v64 = static_cast<double*>(lv.d_V) + in.soff;
+                hipLaunchKernelGGL(k_widen_real<float>, dim3((unsigned)grid_blocks(in.nsite)), dim3(256), 0,
+                                   b->stream, static_cast<const float*>(Vi), v64, in.nsite);
+                SFD_HIP(hipGetLastError());
+                Vi = v64;
             }
-            void*& dev = b->phase[2 * (size_t)i + ax];
-            if (!dev) SFD_HIP(hipMalloc(&dev, n * sizeof(cx<R>)));
-            SFD_TRY(upload_sync(dev, h.data(), n * sizeof(cx<R>)));
         }
-        for (int k = 0; k < sfd_batch::kKinds; ++k) {
-            const int ax = sfd_batch::kind_axis(k);
+        hi[i].psi = field;
+        hi[i].snap = static_cast<char*>(lv.d_snap) + (size_t)in.soff * sizeof(cx<S>);
+        hi[i].partial = lv.d_partial + lv.poff[i];
+        hi[i].nsite = in.nsite;
+        hi[i].dv = in.d[0] * in.d[1] * (in.ndim == 3 ? in.d[2] : 1.0);
+        int rows[3] = {1, 1, 1};
+        for (int ax = 0; ax < in.ndim; ++ax) {
+            rows[ax] = (Om != 0.0 && ax < 2) ? in.n[ax == 0 ? 1 : 0] : 1;
+            const size_t bytes = (size_t)rows[ax] * in.n[ax] * sizeof(cx<R>);
+            void*& dev = lv.phase[3 * (size_t)i + ax];
+            if (dev && lv.phase_bytes[3 * (size_t)i + ax] != bytes) {
+                SFD_HIP(hipStreamSynchronize(b->stream));
+                SFD_HIP(hipFree(dev));
+                dev = nullptr;
+            }
+            if (!dev) {
+                SFD_HIP(hipMalloc(&dev, bytes));
+                lv.phase_bytes[3 * (size_t)i + ax] = bytes;
+            }
+            const long long tot = (long long)rows[ax] * in.n[ax];
+            hipLaunchKernelGGL(k_build_phase<R>, dim3((unsigned)std::min<long long>((tot + 255) / 256, 1024)),
+                               dim3(256), 0, b->stream, static_cast<cx<R>*>(dev), (const double*)in.geo[ax],
+                               (const double*)(in.geo[ax == 0 ? 1 : 0] + in.n[ax == 0 ? 1 : 0]), in.n[ax], rows[ax],
+                               ax, tau, cD, Om, 1);
+            SFD_HIP(hipGetLastError());
+        }
+        int norm_axis = 0;
+        for (size_t k = 0; k < G.kinds.size(); ++k) {
+            const auto& kd = G.kinds[k];
+            const int ax = kd.axis;
             SweepArgs a;
             std::memset(&a, 0, sizeof a);
-            a.psi = psi[in.user];
-            a.V = V ? V[in.user] : nullptr;
-            a.phase = b->phase[2 * (size_t)i + ax];
-            a.tw = G.tw[ax];
+            a.psi = field;
+            a.V = Vi;
+            a.phase = lv.phase[3 * (size_t)i + ax];
+            a.phase_row_stride = rows[ax] > 1 ? in.n[ax] : 0;
+            if (ax == 0) a.row_mod = rows[ax] > 1 ? in.n[1] : 0;
+            else a.row_from_col = (ax == 1 && rows[ax] > 1) ? 1 : 0;
+            a.tw = G.tw[L][ax];
             a.scale = b->d_scale + i;
             a.beta = beta[in.user];
             a.dt = dt;
-            batch_geom(a, in, G, ax);
-            // x (rescale), [y, nonlinear, y], x (+ norm partials): the
-            // folded Strang step of split_impl
-            a.flags = kFlagImag;
-            if (k == 0) a.flags |= kFlagScale;
-            if (k == 1) a.flags |= kFlagNL;
-            if (k == 2) {
-                a.flags |= kFlagNorm;
-                a.partial = b->d_partial + poff;
+            a.flags = kFlagImag | kd.flags;
+            if (kd.flags & kFlagNorm) {
+                a.partial = hi[i].partial;
+                norm_axis = ax;
             }
-            desc[(size_t)k * ni + i] = a;
+            batch_geom(a, in, G, L, ax);
+            desc[k * (size_t)ni + i] = a;
         }
-        poff += G.ntiles[0];
-        soff += in.nsite;
+        hi[i].nparts = G.ntiles[L][norm_axis];
     }
-    SFD_TRY(upload_sync(b->d_inst, hi.data(), hi.size() * sizeof(BatchInstDev)));
-    SFD_TRY(upload_sync(b->d_desc, desc.data(), desc.size() * sizeof(SweepArgs)));
+    SFD_HIP(hipMemcpyAsync(lv.d_inst, hi.data(), hi.size() * sizeof(BatchInstDev), hipMemcpyHostToDevice, b->stream));
+    SFD_HIP(hipMemcpyAsync(lv.d_desc, desc.data(), desc.size() * sizeof(SweepArgs), hipMemcpyHostToDevice,
+                           b->stream));
+    SFD_HIP(hipStreamSynchronize(b->stream));  // the host vectors go out of scope
     return SFD_OK;
 }
 
-// Enqueue k imaginary-time Strang steps of every unfinished instance; the
-// last step is a convergence check when `check` is set.  Each size group
-// runs its k steps on its own stream (forked from and joined back to the
-// batch stream), so the groups' small launches overlap.
+// One imaginary-time Strang step of the unfinished instances of group g at
+// level L on stream st: the group's sweeps, the per-instance renormalisation
+// factor and, on a check step, the snapshot before and ||Delta psi|| after
+// (with the escalation policy of k_batch_diff_finalize when `escalate`).
 template <typename P>
-int batch_enqueue(sfd_batch* b, int k, bool check, double tol)
+int batch_level_step(sfd_batch* b, int L, int g, hipStream_t st, bool check, double tol, bool escalate)
 {
     using S = typename P::S;
-    const int ni = b->ninst;
+    const auto& G = b->groups[g];
+    const auto& lv = b->lv[L];
+    const int ni = b->ninst, f = G.first;
     constexpr int KB = sfd_batch::kDiffBlocks;
-    SFD_HIP(hipEventRecord(b->fork, b->stream));
-    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-        const auto& G = b->groups[gi];
-        hipStream_t st = b->gstream[gi];
-        SFD_HIP(hipStreamWaitEvent(st, b->fork, 0));
-        const int f = G.first;
-        for (int s = 0; s < k; ++s) {
-            const bool chk = check && s == k - 1;
-            if (chk) {
-                hipLaunchKernelGGL(k_batch_snapshot<S>, dim3(KB, G.count), dim3(256), 0, st, b->d_inst + f,
-                                   b->d_scale + f, b->d_snap_scale + f, b->d_done + f);
-                SFD_HIP(hipGetLastError());
-            }
-            for (int kind = 0; kind < sfd_batch::kKinds; ++kind) {
-                const int ax = sfd_batch::kind_axis(kind);
-                const VariantInfo& vi = G.vi[ax];
-                dim3 grid((unsigned)G.ntiles[ax], 1, (unsigned)G.count);
-                dim3 block(vi.ts * vi.g * vi.n2 * vi.h);
-                if (!launch_batch<P>(G.n[ax], ax == 0, G.var[ax], grid, block, st,
-                                     b->d_desc + (size_t)kind * ni + f, b->d_done + f, kind == 1))
-                    return fail(SFD_ERR_SHAPE, "batch: no kernel for line length %d", G.n[ax]);
-                SFD_HIP(hipGetLastError());
-            }
-            hipLaunchKernelGGL(k_batch_finalize, dim3(G.count), dim3(256), 0, st, b->d_inst + f, b->d_scale + f,
-                               b->d_done + f, b->d_steps + f);
-            SFD_HIP(hipGetLastError());
-            if (chk) {
-                hipLaunchKernelGGL(k_batch_diff<S>, dim3(KB, G.count), dim3(256), 0, st, b->d_inst + f,
-                                   b->d_scale + f, b->d_snap_scale + f, b->d_done + f, b->d_dpart + (size_t)f * KB);
-                SFD_HIP(hipGetLastError());
-                hipLaunchKernelGGL(k_batch_diff_finalize, dim3(G.count), dim3(256), 0, st,
-                                   b->d_dpart + (size_t)f * KB, KB, b->d_inst + f, b->d_dpsi + f, b->d_done + f,
-                                   tol);
-                SFD_HIP(hipGetLastError());
-            }
-        }
-        SFD_HIP(hipEventRecord(b->gdone[gi], st));
-        SFD_HIP(hipStreamWaitEvent(b->stream, b->gdone[gi], 0));
+    if (check) {
+        hipLaunchKernelGGL(k_batch_snapshot<S>, dim3(KB, G.count), dim3(256), 0, st, lv.d_inst + f, b->d_scale + f,
+                           b->d_snap_scale + f, lv.d_done + f);
+        SFD_HIP(hipGetLastError());
+    }
+    for (size_t k = 0; k < G.kinds.size(); ++k) {
+        const int ax = G.kinds[k].axis;
+        const VariantInfo& vi = G.vi[L][ax];
+        dim3 grid((unsigned)G.ntiles[L][ax], 1, (unsigned)G.count);
+        dim3 block(vi.ts * vi.g * vi.n2 * vi.h);
+        if (!launch_batch<P>(G.n[ax], ax == 0, G.var[L][ax], grid, block, st, lv.d_desc + k * (size_t)ni + f,
+                             lv.d_done + f, G.kinds[k].twice))
+            return fail(SFD_ERR_SHAPE, "batch: no kernel for line length %d", G.n[ax]);
+        SFD_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_batch_finalize, dim3(G.count), dim3(256), 0, st, lv.d_inst + f, b->d_scale + f,
+                       lv.d_done + f, b->d_steps + f);
+    SFD_HIP(hipGetLastError());
+    if (check) {
+        hipLaunchKernelGGL(k_batch_diff<S>, dim3(KB, G.count), dim3(256), 0, st, lv.d_inst + f, b->d_scale + f,
+                           b->d_snap_scale + f, lv.d_done + f, b->d_dpart + (size_t)f * KB);
+        SFD_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_batch_diff_finalize, dim3(G.count), dim3(256), 0, st, b->d_dpart + (size_t)f * KB, KB,
+                           lv.d_inst + f, b->d_dpsi + f, b->d_dprev + f, lv.d_done + f, tol, b->d_esc + f,
+                           escalate ? 1 : 0);
+        SFD_HIP(hipGetLastError());
     }
     return SFD_OK;
 }
 
-// Batch autotuner: for every size group and axis with more than one compiled
-// variant, time the identity round trip (tau = 0) of k_batch_sweep (the
-// double round trip on y, as in a step) over the
-// group's instances on the snapshot buffer filled with unit-modulus
-// pseudo-random data (the fields are not touched), 2 warm-up launches then
-// the best of 3 batches of 8, and keep the
-// fastest.  times[16 g + 8 ax + v] = ms per launch of variant v (-1 if none).
+// Escalation of the instances of group g flagged at this check (level 0 in
+// precision P to the FP64 level).
 template <typename P>
-int batch_autotune_impl(sfd_batch* b, float* times)
+int batch_escalate(sfd_batch* b, int g, hipStream_t st)
+{
+    using S = typename P::S;
+    const auto& G = b->groups[g];
+    const int f = G.first;
+    constexpr int KB = sfd_batch::kDiffBlocks;
+    double* sq = b->d_sq + (size_t)f * 2 * KB;
+    hipLaunchKernelGGL(k_batch_sq_partials<S>, dim3(KB, G.count), dim3(256), 0, st, b->lv[0].d_inst + f,
+                       b->d_esc + f, sq);
+    SFD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_batch_escalate<S>, dim3(KB, G.count), dim3(256), 0, st, b->lv[0].d_inst + f,
+                       b->lv[1].d_inst + f, b->d_scale + f, b->d_esc + f, b->d_real + f, (const double*)sq, KB,
+                       (const double*)(b->d_phi0 + f));
+    SFD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_batch_escalate_flags, dim3((G.count + 255) / 256), dim3(256), 0, st, b->d_scale + f,
+                       b->d_esc + f, b->lv[1].d_done + f, b->d_dprev + f, G.count);
+    SFD_HIP(hipGetLastError());
+    return SFD_OK;
+}
+
+// Start of an escalating solve (level 0 in precision P): the dominant phase
+// of every initial state, the phase the escalation restores.
+template <typename P>
+int batch_phase0(sfd_batch* b)
+{
+    using S = typename P::S;
+    constexpr int KB = sfd_batch::kDiffBlocks;
+    hipLaunchKernelGGL(k_batch_sq_partials<S>, dim3(KB, b->ninst), dim3(256), 0, b->stream, b->lv[0].d_inst,
+                       (const int*)nullptr, b->d_sq);
+    SFD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_batch_phase0, dim3((b->ninst + 255) / 256), dim3(256), 0, b->stream, (const double*)b->d_sq,
+                       KB, b->d_phi0, b->ninst);
+    SFD_HIP(hipGetLastError());
+    return SFD_OK;
+}
+
+// End of a solve (level 0 in precision P): every instance's final state
+// times its pending scale into the caller's field, and in FP64 to the
+// caller's outputs.
+template <typename P>
+int batch_finish(sfd_batch* b)
+{
+    using S = typename P::S;
+    const BatchInstDev* hi = b->lv[1].prec ? b->lv[1].d_inst : nullptr;
+    hipLaunchKernelGGL(k_batch_finish<S>, dim3(sfd_batch::kDiffBlocks, b->ninst), dim3(256), 0, b->stream,
+                       b->lv[0].d_inst, hi, b->d_scale, b->d_esc, (void* const*)b->d_out64);
+    SFD_HIP(hipGetLastError());
+    return SFD_OK;
+}
+
+// Batch autotuner (level 0): for every size group and axis with more than
+// one compiled variant, time the identity round trip (tau = 0) of
+// k_batch_sweep over the group's instances on the level's snapshot buffer
+// filled with unit-modulus pseudo-random data (the fields are not touched),
+// as the double round trip on the axis whose flows the step folds (y in 2D,
+// z in 3D); 2 warm-up launches, then the best of 3 batches of 8, and keep
+// the fastest.  times[24 g + 8 ax + v] = ms per launch (-1 if none).
+template <typename P>
+int batch_autotune(sfd_batch* b, float* times)
 {
     using R = typename P::R;
     using S = typename P::S;
+    auto& lv = b->lv[0];
     const int ni = b->ninst;
     if (times)
-        for (int i = 0; i < 16 * ni; ++i) times[i] = -1.0f;
-    batch_drop_graph(b);
-    long long sites = 0;
-    for (auto& in : b->inst) sites += in.nsite;
+        for (int i = 0; i < 24 * ni; ++i) times[i] = -1.0f;
+    batch_drop_graphs(b);
     SFD_HIP(hipEventRecord(b->ev, b->user_stream));
     SFD_HIP(hipStreamWaitEvent(b->stream, b->ev, 0));
-    hipLaunchKernelGGL(k_fill_phases<S>, dim3((unsigned)grid_blocks(sites)), dim3(256), 0, b->stream,
-                       static_cast<cx<S>*>(b->d_snap), sites);
+    hipLaunchKernelGGL(k_fill_phases<S>, dim3((unsigned)grid_blocks(b->sites)), dim3(256), 0, b->stream,
+                       static_cast<cx<S>*>(lv.d_snap), b->sites);
     SFD_HIP(hipGetLastError());
-    SFD_HIP(hipMemsetAsync(b->d_done, 0, ni * sizeof(int), b->stream));
+    hipLaunchKernelGGL(k_batch_reset, dim3((ni + 255) / 256), dim3(256), 0, b->stream, b->d_scale, lv.d_done,
+                       (int*)nullptr, b->d_steps, b->d_dpsi, b->d_dprev, b->d_esc, ni);
+    SFD_HIP(hipGetLastError());
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::vector<void*> tmp;  // identity phase tables
+    std::vector<SweepArgs> desc;
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(b->stream);
         if (e0) (void)hipEventDestroy(e0);
@@ -982,7 +1055,8 @@ int batch_autotune_impl(sfd_batch* b, float* times)
         SFD_HIP(hipEventCreate(&e1));
         for (size_t gi = 0; gi < b->groups.size(); ++gi) {
             auto& G = b->groups[gi];
-            for (int ax = 0; ax < 2; ++ax) {
+            const int fold_axis = G.ndim == 2 ? 1 : 2;
+            for (int ax = 0; ax < G.ndim; ++ax) {
                 const int nv = n_variants<P>(G.n[ax], ax == 0);
                 if (nv <= 1) continue;
                 const int n = G.n[ax];
@@ -994,28 +1068,26 @@ int batch_autotune_impl(sfd_batch* b, float* times)
                 float best = 1e30f;
                 int best_v = 0;
                 for (int v = 0; v < nv && v < 8; ++v) {
-                    SFD_TRY(batch_group_variant<P>(b, G, ax, v));
-                    std::vector<SweepArgs> desc(G.count);
-                    long long soff = 0;
-                    for (int i = 0; i < G.first; ++i) soff += b->inst[i].nsite;
+                    SFD_TRY(batch_level_variant<P>(b, 0, (int)gi, ax, v));
+                    desc.assign(G.count, SweepArgs{});
                     for (int k = 0; k < G.count; ++k) {
                         const auto& in = b->inst[G.first + k];
                         SweepArgs& a = desc[k];
                         std::memset(&a, 0, sizeof a);
-                        a.psi = static_cast<char*>(b->d_snap) + soff * sizeof(cx<S>);
+                        a.psi = static_cast<char*>(lv.d_snap) + (size_t)in.soff * sizeof(cx<S>);
                         a.phase = ph;
-                        a.tw = G.tw[ax];
+                        a.tw = G.tw[0][ax];
                         a.flags = kFlagImag;
-                        batch_geom(a, in, G, ax);
-                        soff += in.nsite;
+                        batch_geom(a, in, G, 0, ax);
                     }
-                    SFD_TRY(upload_sync(b->d_desc, desc.data(), desc.size() * sizeof(SweepArgs)));
-                    const VariantInfo& vi = G.vi[ax];
-                    dim3 grid((unsigned)G.ntiles[ax], 1, (unsigned)G.count);
+                    SFD_HIP(hipMemcpyAsync(lv.d_desc, desc.data(), desc.size() * sizeof(SweepArgs),
+                                           hipMemcpyHostToDevice, b->stream));
+                    const VariantInfo& vi = G.vi[0][ax];
+                    dim3 grid((unsigned)G.ntiles[0][ax], 1, (unsigned)G.count);
                     dim3 block(vi.ts * vi.g * vi.n2 * vi.h);
-                    // the y sweep of a step is the double round trip
                     auto launch = [&]() {
-                        return launch_batch<P>(n, ax == 0, v, grid, block, b->stream, b->d_desc, b->d_done, ax == 1);
+                        return launch_batch<P>(n, ax == 0, v, grid, block, b->stream, lv.d_desc, lv.d_done,
+                                               ax == fold_axis);
                     };
                     for (int w = 0; w < 2; ++w)
                         if (!launch()) return fail(SFD_ERR_SHAPE, "batch autotune: no kernel %d for n = %d", v, n);
@@ -1030,13 +1102,13 @@ int batch_autotune_impl(sfd_batch* b, float* times)
                         SFD_HIP(hipEventElapsedTime(&ms, e0, e1));
                         t_best = std::min(t_best, ms / 8.0f);
                     }
-                    if (times) times[16 * gi + 8 * ax + v] = t_best;
+                    if (times) times[24 * gi + 8 * ax + v] = t_best;
                     if (t_best < best) {
                         best = t_best;
                         best_v = v;
                     }
                 }
-                SFD_TRY(batch_group_variant<P>(b, G, ax, best_v));
+                SFD_TRY(batch_level_variant<P>(b, 0, (int)gi, ax, best_v));
             }
         }
         return SFD_OK;
@@ -1044,69 +1116,6 @@ int batch_autotune_impl(sfd_batch* b, float* times)
     const int rc = run();
     cleanup();
     return rc;
-}
-
-template <typename P>
-int batch_run(sfd_batch* b, void* const* psi, const void* const* V, const double* beta, double cD, double dt,
-              int64_t max_steps, double tol, int check_every, int use_graph, int64_t* steps_out, double* dpsi_out)
-{
-    using S = typename P::S;
-    const int ni = b->ninst;
-    SFD_TRY(batch_bind<P>(b, psi, V, beta, cD, dt));
-    // order after the caller's pending work
-    SFD_HIP(hipEventRecord(b->ev, b->user_stream));
-    SFD_HIP(hipStreamWaitEvent(b->stream, b->ev, 0));
-    hipLaunchKernelGGL(k_batch_reset, dim3((ni + 255) / 256), dim3(256), 0, b->stream, b->d_scale, b->d_done,
-                       b->d_steps, b->d_dpsi, ni, 1);
-    SFD_HIP(hipGetLastError());
-    batch_drop_graph(b);
-    std::vector<int> done(ni, 0);
-    int64_t left = max_steps;
-    while (left > 0) {
-        const int k = (int)(left < check_every ? left : check_every);
-        if (use_graph && k == check_every && k > 0) {
-            if (!b->gexec) {
-                SFD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-                int rc = batch_enqueue<P>(b, k, true, tol);
-                hipGraph_t g = nullptr;
-                hipError_t e = hipStreamEndCapture(b->stream, &g);
-                if (rc != SFD_OK) {
-                    if (g) (void)hipGraphDestroy(g);
-                    return rc;
-                }
-                if (e != hipSuccess) return fail(SFD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-                b->graph = g;
-                SFD_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
-                b->graph_k = k;
-            }
-            SFD_HIP(hipGraphLaunch(b->gexec, b->stream));
-        } else {
-            SFD_TRY(batch_enqueue<P>(b, k, true, tol));
-        }
-        left -= k;
-        SFD_HIP(hipMemcpyAsync(done.data(), b->d_done, ni * sizeof(int), hipMemcpyDeviceToHost, b->stream));
-        SFD_HIP(hipStreamSynchronize(b->stream));
-        bool all = true;
-        for (int i = 0; i < ni; ++i) all = all && done[i];
-        if (all) break;
-    }
-    // apply the pending renormalisation, leave normalised fields
-    hipLaunchKernelGGL(k_batch_apply_scale<S>, dim3(sfd_batch::kDiffBlocks, ni), dim3(256), 0, b->stream, b->d_inst,
-                       b->d_scale);
-    SFD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_batch_reset, dim3((ni + 255) / 256), dim3(256), 0, b->stream, b->d_scale, b->d_done,
-                       b->d_steps, b->d_dpsi, ni, 0);
-    SFD_HIP(hipGetLastError());
-    std::vector<long long> st(ni);
-    std::vector<double> dp(ni);
-    SFD_HIP(hipMemcpyAsync(st.data(), b->d_steps, ni * sizeof(long long), hipMemcpyDeviceToHost, b->stream));
-    SFD_HIP(hipMemcpyAsync(dp.data(), b->d_dpsi, ni * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-    SFD_HIP(hipStreamSynchronize(b->stream));
-    for (int i = 0; i < ni; ++i) {
-        if (steps_out) steps_out[b->inst[i].user] = st[i];
-        if (dpsi_out) dpsi_out[b->inst[i].user] = dp[i];
-    }
-    return SFD_OK;
 }
 
 // Wrappers with the signatures of PrecOps.
@@ -1140,9 +1149,14 @@ PrecOps make_ops()
     o.n_variants = &n_variants<P>;
     o.build_phase_into = &launch_build_phase<P>;
     o.table_entry_bytes = sizeof(cx<typename P::R>);
-    o.batch_setup = &batch_setup<P>;
-    o.batch_autotune = &batch_autotune_impl<P>;
-    o.batch_run = &batch_run<P>;
+    o.batch_level_setup = &batch_level_setup<P>;
+    o.batch_level_variant = &batch_level_variant<P>;
+    o.batch_level_bind = &batch_level_bind<P>;
+    o.batch_level_step = &batch_level_step<P>;
+    o.batch_escalate = &batch_escalate<P>;
+    o.batch_phase0 = &batch_phase0<P>;
+    o.batch_finish = &batch_finish<P>;
+    o.batch_autotune = &batch_autotune<P>;
     return o;
 }
 

@@ -203,20 +203,51 @@ struct sfd_ctx {
 
 // ------------------------------------------------------------ batch solver
 struct sfd_batch {
+    static constexpr int kDiffBlocks = 64;
+    static constexpr int kMaxKinds = 5;  // sweeps of one imaginary-time Strang step
+    static constexpr int kLevels = 2;    // 0: the batch's precision, 1: FP64 after escalation
     struct Inst {
-        int n[2];
-        double lo[2], hi[2], d[2];
+        int ndim;
+        int n[3];
+        double lo[3], hi[3], d[3];
         long long nsite;
+        long long soff;  // offset of the instance's sites in the batch's own buffers
         int group;
-        int user;  // index in the caller's ordering
+        int user;        // index in the caller's ordering
+        double* geo[3];  // device FP64 wavenumbers [0, n) and coordinates [n, 2n) per axis
+    };
+    // One sweep of a step: axis, fused flags (kFlagScale / kFlagNL /
+    // kFlagNorm) and whether it is the double round trip [d N d].
+    struct Kind {
+        int axis, flags;
+        bool twice;
     };
     struct Group {
-        int n[2];
+        int ndim;
+        int n[3];
         int first = 0, count = 0;
-        sfd::VariantInfo vi[2] = {};
-        int var[2] = {0, 0};  // launch variant per axis (sfd_batch_autotune)
-        void* tw[2] = {nullptr, nullptr};
-        long long ntiles[2] = {0, 0};
+        // per level and axis: launch variant, its twiddles, tiles and x-tiles
+        sfd::VariantInfo vi[kLevels][3] = {};
+        int var[kLevels][3] = {};
+        void* tw[kLevels][3] = {};
+        long long ntiles[kLevels][3] = {};
+        int ntx[kLevels][3] = {};
+        std::vector<Kind> kinds;  // the step of the current solve
+    };
+    // Device state of one precision level.
+    struct Level {
+        int prec = 0;                          // SFD_F32 / SFD_F32X / SFD_F64; 0 = not set up
+        sfd::BatchInstDev* d_inst = nullptr;   // per instance: field, snapshot, partials
+        sfd::SweepArgs* d_desc = nullptr;      // [kMaxKinds][ninst]
+        int* d_done = nullptr;
+        double* d_partial = nullptr;
+        long long parts_per_inst_max = 0;      // partials reserved per instance (see batch setup)
+        std::vector<long long> poff;           // per instance: offset of its partials
+        void* d_snap = nullptr;                // snapshots, in the level's storage precision
+        void* d_field = nullptr;               // level 1: the FP64 fields of escalated instances
+        void* d_V = nullptr;                   // level 1: FP64 copies of the potentials
+        std::vector<void*> phase;              // [ninst * 3] phase tables of the current solve
+        std::vector<size_t> phase_bytes;
     };
     int ninst = 0;
     int prec = SFD_F32;
@@ -231,24 +262,27 @@ struct sfd_batch {
     hipEvent_t fork = nullptr;
     std::vector<Inst> inst;        // internal (group-major) order
     std::vector<Group> groups;
-    sfd::BatchInstDev* d_inst = nullptr;
+    long long sites = 0;
+    Level lv[kLevels];
     double* d_scale = nullptr;
     double* d_snap_scale = nullptr;
     double* d_dpsi = nullptr;
-    int* d_done = nullptr;
+    double* d_dprev = nullptr;
+    int* d_esc = nullptr;
+    int* d_real = nullptr;         // per instance: 1 when the Hamiltonian is real (no rotation)
+    double* d_sq = nullptr;        // escalation: per-workgroup sums of psi^2, [ninst][2][kDiffBlocks]
+    double* d_phi0 = nullptr;      // per instance: dominant phase of the initial state
     long long* d_steps = nullptr;
-    double* d_partial = nullptr;
     double* d_dpart = nullptr;
-    void* d_snap = nullptr;
-    sfd::SweepArgs* d_desc = nullptr;   // [4 kinds][ninst]
-    std::vector<void*> phase;      // per instance, per axis
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_k = 0;
-    static constexpr int kDiffBlocks = 64;
-    // sweeps of one imaginary-time Strang step: x, [y N y], x
-    static constexpr int kKinds = 3;
-    static int kind_axis(int kind) { return kind == 1 ? 1 : 0; }
+    void** d_out64 = nullptr;      // per solve: the caller's FP64 outputs (internal order)
+    // hipGraphs of one chunk, keyed by the active (group, level) mask
+    struct CachedGraph {
+        unsigned long long mask;
+        int k;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<CachedGraph> graphs;
 };
 
 namespace sfd {
@@ -322,8 +356,8 @@ void release(sfd_ctx* c);
 int timer_begin(sfd_ctx* c, int slot);
 int timer_end(sfd_ctx* c);
 int timer_drain(sfd_ctx* c);
-void batch_geom(SweepArgs& a, const sfd_batch::Inst& in, const sfd_batch::Group& G, int ax);
-void batch_drop_graph(sfd_batch* b);
+void batch_geom(SweepArgs& a, const sfd_batch::Inst& in, const sfd_batch::Group& G, int level, int ax);
+void batch_drop_graphs(sfd_batch* b);
 void batch_release(sfd_batch* b);
 
 // ------------------------------------------------ per-precision routines
@@ -347,11 +381,17 @@ struct PrecOps {
     int (*build_phase_into)(const sfd_ctx* c, hipStream_t stream, void* dev, int axis, double tau, double cD,
                             double Om, int imag, int rows);
     size_t table_entry_bytes;  // bytes of one phase-table entry
-    int (*batch_setup)(sfd_batch* b);
+    // batch level routines (level 0: the batch's precision; 1: FP64)
+    int (*batch_level_setup)(sfd_batch* b, int level);
+    int (*batch_level_variant)(sfd_batch* b, int level, int group, int axis, int idx);
+    int (*batch_level_bind)(sfd_batch* b, int level, void* const* psi, const void* const* V, const double* beta,
+                            const double* Omega, double cD, double dt);
+    int (*batch_level_step)(sfd_batch* b, int level, int group, hipStream_t st, bool check, double tol,
+                            bool escalate);
+    int (*batch_escalate)(sfd_batch* b, int group, hipStream_t st);
+    int (*batch_phase0)(sfd_batch* b);
+    int (*batch_finish)(sfd_batch* b);
     int (*batch_autotune)(sfd_batch* b, float* times);
-    int (*batch_run)(sfd_batch* b, void* const* psi, const void* const* V, const double* beta, double cD, double dt,
-                     int64_t max_steps, double tol, int check_every, int use_graph, int64_t* steps_out,
-                     double* dpsi_out);
 };
 
 extern const PrecOps kOpsF32, kOpsF64, kOpsF32X;

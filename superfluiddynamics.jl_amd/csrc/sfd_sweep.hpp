@@ -730,9 +730,11 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
 // Batched round trip over independent fields: blockIdx.z selects the
 // instance, whose descriptor (field, potential, tables, strides, flags,
 // pending scale, partial sums) sits in descs[]; instances whose done flag is
-// set are skipped.  All instances of one launch share N1, N2, H and the
-// workgroup shape; grid.x*grid.y covers the largest tile count.  TWICE runs
-// K(tau) N K(tau) on the lines (the folded Strang step of kRoundTripTwice).
+// set are skipped.  All instances of one launch share their extents, N1, N2,
+// H and the workgroup shape; grid.x*grid.y covers the tile count.  TWICE
+// runs K(tau) N K(tau) on the lines (the folded Strang step of
+// kRoundTripTwice).  Imaginary-time multipliers only (FP32: real decay
+// tables).
 template <typename S, typename R, int N1, int N2, int TS, int G, bool TWICE = false, int H = 1, bool TL = false>
 __global__ __launch_bounds__(TS * G * N2 * H) void k_batch_sweep(const SweepArgs* __restrict__ descs,
                                                                  const int* __restrict__ done)
@@ -767,14 +769,14 @@ __global__ __launch_bounds__(TS * G * N2 * H) void k_batch_sweep(const SweepArgs
     if (a.flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
     double nacc = 0.0;
     const cx<R>* ph = row_phase<R>(a, gm);
-    if constexpr (TL) {  // tables from LDS (batch phase tables have one row)
+    if constexpr (TL) {  // tables from LDS; a phase table with rows (rotation) stays in global memory
         constexpr int NTW = kTwiddles<M, H>;
         __shared__ cx<R> tabs[NTW + N];
         stage_tables<R, M, H, Sh::NT>(tabs, a, tid);
         twA = tabs + t;
         twB = tabs + M + t;
         twC = tabs + 2 * M + t;
-        ph = tabs + NTW;
+        if (a.phase_row_stride == 0) ph = tabs + NTW;
     }
     if constexpr (TWICE) {
         SweepArgs a1 = a;

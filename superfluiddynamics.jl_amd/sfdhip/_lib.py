@@ -27,6 +27,8 @@ SFD_REAL_TIME = 0
 SFD_IMAG_TIME = 1
 SFD_FORWARD = -1
 SFD_BACKWARD = 1
+SFD_BATCH_GRAPH = 1
+SFD_BATCH_ESCALATE = 2
 
 #: every symbol include/sfd.h declares
 EXPORTS = (
@@ -36,6 +38,7 @@ EXPORTS = (
     "sfd_autotune", "sfd_ctx_get_variant", "sfd_ctx_set_variant",
     "sfd_batch_create", "sfd_batch_destroy", "sfd_batch_imag_run", "sfd_batch_autotune", "sfd_adjoint",
     "sfd_project", "sfd_inner", "sfd_ctx_time_sweeps", "sfd_ctx_sweep_times", "sfd_split_step_noalloc",
+    "sfd_batch_create_nd", "sfd_batch_imag_solve",
 )
 
 
@@ -102,6 +105,9 @@ def load():
                                 c_int),
         "sfd_ctx_set_variant": ([vp, c_int, c_int], c_int),
         "sfd_batch_create": ([ctypes.POINTER(vp), c_int, i64p, dp, c_int, vp], c_int),
+        "sfd_batch_create_nd": ([ctypes.POINTER(vp), c_int, ctypes.POINTER(c_int), i64p, dp, c_int, vp], c_int),
+        "sfd_batch_imag_solve": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), dp, dp, c_dbl, c_dbl, c_i64, c_dbl,
+                                  c_int, c_int, i64p, dp, ctypes.POINTER(vp), ctypes.POINTER(c_int)], c_int),
         "sfd_batch_destroy": ([vp], c_int),
         "sfd_batch_autotune": ([vp, ctypes.POINTER(ctypes.c_float)], c_int),
         "sfd_adjoint": ([vp, vp, vp, vp, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_i64, vp, dp, dp, vp], c_int),

@@ -582,13 +582,13 @@ def computeDerivatives(gf: GradientField, p: Plan, ϕt) -> None:
 
 # --------------------------------------------------- batched ground states
 class BatchImagSolver:
-    """Many independent 2D imaginary-time ground-state solves advanced
-    together (sfd_batch_imag_run): fields of equal extents share every sweep
-    launch, each field keeps its own β, V, FP64 renormalisation factor, step
-    count and convergence flag, and chunks of steps replay as a hipGraph.
+    """Many independent 2D and 3D imaginary-time ground-state solves advanced
+    together (sfd_batch_imag_solve): fields of equal shape share every sweep
+    launch, each field keeps its own β, Ω, V, FP64 renormalisation factor,
+    step count and convergence flag, and chunks of steps replay as hipGraphs.
 
-    ``fields`` is a list of 2D Fields of one precision; ``params`` a list of
-    GrossPitaevskiiParameters (one coeffΔ for all, Ω = 0)."""
+    ``fields`` is a list of 2D or 3D Fields of one precision; ``params`` a
+    list of GrossPitaevskiiParameters (one coeffΔ for all; Ω per field)."""
 
     def __init__(self, fields, params, Δt):
         _require_gpu()
@@ -596,54 +596,74 @@ class BatchImagSolver:
             raise L.ArgError(L.SFD_ERR_ARG, "one parameter set per field")
         g0 = fields[0].g
         for f in fields:
-            if f.g.ndims != 2:
-                raise L.ArgError(L.SFD_ERR_ARG, "batched solves are 2D")
+            if f.g.ndims not in (2, 3):
+                raise L.ArgError(L.SFD_ERR_ARG, "batched solves are 2D or 3D")
             if (f.g.FT, f.g.compute) != (g0.FT, g0.compute):
                 raise L.ArgError(L.SFD_ERR_ARG, "all fields of a batch share one precision")
         cds = {p.coeffΔ for p in params}
-        if len(cds) != 1 or any(p.Ω != 0.0 for p in params):
-            raise L.ArgError(L.SFD_ERR_ARG, "batched solves need one coeffΔ and Ω = 0")
+        if len(cds) != 1:
+            raise L.ArgError(L.SFD_ERR_ARG, "batched solves need one coeffΔ")
         self.fields, self.params, self.Δt = list(fields), list(params), float(Δt)
         self.coeffΔ = cds.pop()
+        self.levels = None
         lib = L.load()
         self.lib = lib
         ni = len(fields)
-        n = (ctypes.c_int64 * (2 * ni))(*[v for f in fields for v in f.g.n])
-        box = (ctypes.c_double * (4 * ni))(*[v for f in fields for a in range(2) for v in (f.g.min[a], f.g.max[a])])
+        nd = (ctypes.c_int * ni)(*[f.g.ndims for f in fields])
+        n = (ctypes.c_int64 * (3 * ni))(*[v for f in fields for v in (list(f.g.n) + [1] * (3 - f.g.ndims))])
+        box = (ctypes.c_double * (6 * ni))(*[v for f in fields for a in range(3)
+                                             for v in ((f.g.min[a], f.g.max[a]) if a < f.g.ndims else (0.0, 1.0))])
         if g0.FT is np.float64:
             prec = L.SFD_F64
         else:
             prec = L.SFD_F32X if g0.compute is np.float64 else L.SFD_F32
         h = ctypes.c_void_p()
-        L.check(lib.sfd_batch_create(ctypes.byref(h), ni, n, box, prec,
-                                     ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        L.check(lib.sfd_batch_create_nd(ctypes.byref(h), ni, nd, n, box, prec,
+                                        ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
         self.h = h
 
     def autotune(self):
-        """Pick the fastest batched-sweep variant per size group and axis
+        """Pick the fastest batched-sweep variant per shape group and axis
         (sfd_batch_autotune; the fields are not touched).  Returns a list of
-        {axis: [ms per launch of each variant]} per size group."""
+        {axis: [ms per launch of each variant]} per shape group."""
         ni = len(self.fields)
-        times = (ctypes.c_float * (16 * ni))()
+        times = (ctypes.c_float * (24 * ni))()
         L.check(self.lib.sfd_batch_autotune(self.h, times))
         out = []
         for g in range(ni):
-            row = {a: [t for t in times[16 * g + 8 * a:16 * g + 8 * a + 8] if t >= 0] for a in range(2)}
-            if any(row.values()):
+            row = {a: [t for t in times[24 * g + 8 * a:24 * g + 8 * a + 8] if t >= 0] for a in range(3)}
+            row = {a: v for a, v in row.items() if v}
+            if row:
                 out.append(row)
         return out
 
-    def run(self, max_steps=20000, tol=1e-9, check_every=100, use_graph=True):
-        """Returns (steps taken per field, last ||Δψ|| per field)."""
+    def run(self, max_steps=20000, tol=1e-9, check_every=100, use_graph=True, escalate=False, fp64_out=None):
+        """Returns (steps taken per field, last ||Δψ|| per field).  With
+        ``escalate`` (FP32 fields) each field finishes in FP64 from its FP32
+        floor; ``fp64_out`` (a list of complex128 device tensors or None per
+        field) receives the final states in FP64.  ``self.levels`` holds 1
+        for the fields that finished in FP64."""
         ni = len(self.fields)
         psi = (ctypes.c_void_p * ni)(*[f.ϕ.data_ptr() for f in self.fields])
         vps = [p.pot.device_ptr() if p.pot is not None else None for p in self.params]
         V = (ctypes.c_void_p * ni)(*[v if v else None for v in vps])
         beta = (ctypes.c_double * ni)(*[p.β for p in self.params])
+        om = (ctypes.c_double * ni)(*[p.Ω for p in self.params])
         steps = (ctypes.c_int64 * ni)()
         dpsi = (ctypes.c_double * ni)()
-        L.check(self.lib.sfd_batch_imag_run(self.h, psi, V, beta, self.coeffΔ, self.Δt, int(max_steps), float(tol),
-                                            int(check_every), int(bool(use_graph)), steps, dpsi))
+        lev = (ctypes.c_int * ni)()
+        out64 = None
+        if fp64_out is not None:
+            if len(fp64_out) != ni:
+                raise L.ArgError(L.SFD_ERR_ARG, "fp64_out needs one entry per field")
+            for t, f in zip(fp64_out, self.fields):
+                if t is not None:
+                    _check_device_tensor(t, "fp64_out", torch.complex128, f.g.shape)
+            out64 = (ctypes.c_void_p * ni)(*[t.data_ptr() if t is not None else None for t in fp64_out])
+        flags = (L.SFD_BATCH_GRAPH if use_graph else 0) | (L.SFD_BATCH_ESCALATE if escalate else 0)
+        L.check(self.lib.sfd_batch_imag_solve(self.h, psi, V, beta, om, self.coeffΔ, self.Δt, int(max_steps),
+                                              float(tol), int(check_every), flags, steps, dpsi, out64, lev))
+        self.levels = np.array(lev[:], dtype=np.int32)
         return np.array(steps[:], dtype=np.int64), np.array(dpsi[:])
 
     def __del__(self):
