@@ -50,8 +50,12 @@ typedef struct sfd_ctx sfd_ctx;
 /* Create a context for one grid: Grid(size, bounds; FT) (Grids.jl:64-129)
  * plus the FFT plan state of Plan(f; t=FFTPlan()) (Plans.jl:821-999):
  * wavenumbers fftfreq(n, 2pi/dx), twiddles, and the workspace of the
- * reductions.  ndim is 2 or 3; n[d] must be a power of two in [2, 1024];
- * box_min/box_max are the bounds ((xmin,xmax),...).  stream is a
+ * reductions.  ndim is 2 or 3; n[d] is a power of two in [2, 2048] or any
+ * other length in [3, 1024] (Grids.jl:64-82 takes any n): lines of other
+ * lengths are transformed by Bluestein's chirp method on the power of two
+ * m >= 2n - 1, at the cost of four m-point FFTs per round trip instead of
+ * two n-point ones (HBM traffic unchanged).  box_min/box_max are the bounds
+ * ((xmin,xmax),...).  stream is a
  * hipStream_t (NULL = the default stream).  Geometry (coordinates, dx,
  * wavenumbers) is kept in FP64 for both precisions. */
 int sfd_ctx_create(sfd_ctx** out, int ndim, const int64_t* n, const double* box_min,

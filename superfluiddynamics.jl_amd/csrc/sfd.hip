@@ -670,9 +670,11 @@ int sfd_ctx_create(sfd_ctx** out, int ndim, const int64_t* n, const double* box_
         return fail(SFD_ERR_ARG, "precision must be SFD_F32, SFD_F64 or SFD_F32X (got %d)", precision);
     for (int a = 0; a < ndim; ++a) {
         const int64_t m = n[a];
-        if (m < 2 || m > 1024 || (m & (m - 1)))
-            return fail(SFD_ERR_SHAPE, "n[%d] = %lld: line lengths must be powers of two in [2, 1024]", a,
-                        (long long)m);
+        const bool pow2 = m >= 2 && (m & (m - 1)) == 0;
+        if (m < 2 || (pow2 && m > 2048) || (!pow2 && m > 1024))
+            return fail(SFD_ERR_SHAPE,
+                        "n[%d] = %lld: line lengths must be powers of two in [2, 2048] or other lengths in [3, 1024]",
+                        a, (long long)m);
         if (!(box_max[a] > box_min[a]) || !std::isfinite(box_min[a]) || !std::isfinite(box_max[a]))
             return fail(SFD_ERR_ARG, "axis %d: bounds must satisfy min < max (Grids.jl:69-76)", a);
     }
@@ -682,6 +684,10 @@ int sfd_ctx_create(sfd_ctx** out, int ndim, const int64_t* n, const double* box_
     c->stream = static_cast<hipStream_t>(stream);
     for (int a = 0; a < ndim; ++a) {
         c->n[a] = (int)n[a];
+        c->blu[a] = (n[a] & (n[a] - 1)) != 0;
+        c->m[a] = c->n[a];
+        if (c->blu[a])
+            for (c->m[a] = 8; c->m[a] < 2 * c->n[a] - 1;) c->m[a] *= 2;
         c->lo[a] = box_min[a];
         c->hi[a] = box_max[a];
         c->d[a] = (box_max[a] - box_min[a]) / (double)n[a];
@@ -970,7 +976,7 @@ int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* sha
     if (!ctx || !index || !count) return fail(SFD_ERR_NULL, "sfd_ctx_get_variant: null argument");
     if (axis < 0 || axis >= ctx->ndim) return fail(SFD_ERR_ARG, "axis %d out of range", axis);
     *index = ctx->var[axis];
-    *count = ops_for(ctx->prec).n_variants(ctx->n[axis], axis == 0);
+    *count = ops_for(ctx->prec).n_variants(ctx, axis);
     if (shape7) {
         shape7[0] = ctx->vi[axis].n1;
         shape7[1] = ctx->vi[axis].n2;

@@ -44,6 +44,45 @@ void fill_twiddles(std::vector<cx<R>>& w, int n, int n1, int n2, int h)
         for (int k = 0; k < m; ++k) put((size_t)2 * m + k, k, n);
 }
 
+// Bluestein tables of a line of n points on m >= 2n - 1 (appended to the
+// m-point twiddles): chirp[p] = exp(-i pi p^2/n) for p < n, zero up to m,
+// then B = DFT_m(g)/m of the wrapped conjugate chirp g[q] = conj(chirp[q])
+// (q < n), g[m - r] = conj(chirp[r]) (0 < r < n).  Angles from p^2 mod 2n
+// (exact integers) in long double; B by a direct sum in long double.
+template <typename R>
+void fill_bluestein(std::vector<cx<R>>& w, int n, int m)
+{
+    const long double pi = 3.14159265358979323846264338327950288L;
+    std::vector<long double> cr(n), ci(n);
+    for (int p = 0; p < n; ++p) {
+        const long long q = ((long long)p * p) % (2LL * n);
+        const long double a = -pi * (long double)q / (long double)n;
+        cr[p] = cosl(a);
+        ci[p] = sinl(a);
+    }
+    const size_t base = w.size();
+    w.resize(base + 2 * (size_t)m);
+    for (int p = 0; p < m; ++p) w[base + p] = p < n ? cx<R>{(R)cr[p], (R)ci[p]} : cx<R>{(R)0, (R)0};
+    std::vector<long double> tr(m), ti(m);  // exp(-2 pi i k/m)
+    for (int k = 0; k < m; ++k) {
+        tr[k] = cosl(-2.0L * pi * (long double)k / (long double)m);
+        ti[k] = sinl(-2.0L * pi * (long double)k / (long double)m);
+    }
+    std::vector<int> idx;
+    std::vector<long double> gr, gi;
+    for (int q = 0; q < n; ++q) { idx.push_back(q); gr.push_back(cr[q]); gi.push_back(-ci[q]); }
+    for (int r = 1; r < n; ++r) { idx.push_back(m - r); gr.push_back(cr[r]); gi.push_back(-ci[r]); }
+    for (int k = 0; k < m; ++k) {
+        long double sr = 0.0L, si = 0.0L;
+        for (size_t e = 0; e < idx.size(); ++e) {
+            const int t = (int)(((long long)k * idx[e]) % m);
+            sr += gr[e] * tr[t] - gi[e] * ti[t];
+            si += gr[e] * ti[t] + gi[e] * tr[t];
+        }
+        w[base + m + k] = {(R)(sr / (long double)m), (R)(si / (long double)m)};
+    }
+}
+
 // One imaginary-time multiplier exp(arg)/n: itself in FP64 tables, and
 // expm1(arg)/n in FP32 tables, whose sweeps form X/n + X expm1(arg)/n.
 template <typename R>
@@ -162,11 +201,21 @@ int launch_sweep(sfd_ctx* c, int axis, SweepArgs a)
     // row; other sweeps run k_sweep.
     const bool persistent = c->vi[axis].pipe && (MODE == kRoundTrip || MODE == kRoundTripImag) &&
                             !(a.flags & (kFlagNL | kFlagScale)) && a.phase_row_stride == 0;
-    Launch L = sweep_geometry(c, axis, a, persistent);
+    const bool blu = c->blu[axis];
+    Launch L = sweep_geometry(c, axis, a, persistent && !blu);
     a.tw = c->tw[axis];
+    a.nlen = c->n[axis];
     if (c->time_sweeps) SFD_TRY(timer_begin(c, rt_twice(MODE) ? 3 : (is_round_trip(MODE) ? axis : 4)));
-    if (!launch_variant<P, MODE>(c->n[axis], axis == 0, c->var[axis], L.grid, L.block, c->stream, a, persistent))
-        return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", c->var[axis], c->n[axis]);
+    bool ok = false;
+    if constexpr (rt_energy(MODE)) {
+        if (blu) return fail(SFD_ERR_SHAPE, "no energy sums fused into Bluestein sweeps (n = %d)", c->n[axis]);
+        ok = launch_variant<P, MODE>(c->n[axis], axis == 0, c->var[axis], L.grid, L.block, c->stream, a, persistent);
+    } else {
+        ok = blu ? launch_blu<P, MODE>(c->m[axis], axis == 0, L.grid, L.block, c->stream, a)
+                 : launch_variant<P, MODE>(c->n[axis], axis == 0, c->var[axis], L.grid, L.block, c->stream, a,
+                                           persistent);
+    }
+    if (!ok) return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", c->var[axis], c->n[axis]);
     SFD_HIP(hipGetLastError());
     if (c->time_sweeps) SFD_TRY(timer_end(c));
     c->last_nblocks = L.nblocks;
@@ -176,15 +225,22 @@ int launch_sweep(sfd_ctx* c, int axis, SweepArgs a)
 // Select launch variant `idx` for `axis` and (re)build the twiddle table of
 // its line split.
 template <typename P>
+int axis_variants(const sfd_ctx* c, int axis)
+{
+    return c->blu[axis] ? 1 : n_variants<P>(c->n[axis], axis == 0);
+}
+
+template <typename P>
 int set_variant(sfd_ctx* c, int axis, int idx)
 {
     using R = typename P::R;
-    const int n = c->n[axis];
+    const int n = c->m[axis];  // the transform length (n itself unless Bluestein)
     VariantInfo vi;
-    if (!variant_info<P>(n, axis == 0, idx, &vi))
+    if ((c->blu[axis] && idx != 0) || !variant_info<P>(n, axis == 0, idx, &vi))
         return fail(SFD_ERR_ARG, "axis %d: variant %d does not exist (%d variants)", axis, idx,
-                    n_variants<P>(n, axis == 0));
-    c->pipe_occ[axis] = vi.pipe ? variant_occupancy<P>(n, axis == 0, idx) : 0;
+                    axis_variants<P>(c, axis));
+    c->pipe_occ[axis] = (vi.pipe && !c->blu[axis]) ? variant_occupancy<P>(n, axis == 0, idx) : 0;
+    if (c->blu[axis]) vi.pipe = 0;
     if (c->tw[axis] && c->vi[axis].n1 == vi.n1 && c->vi[axis].n2 == vi.n2 && c->vi[axis].h == vi.h) {
         c->var[axis] = idx;
         c->vi[axis] = vi;
@@ -192,6 +248,7 @@ int set_variant(sfd_ctx* c, int axis, int idx)
     }
     std::vector<cx<R>> w;
     fill_twiddles<R>(w, n, vi.n1, vi.n2, vi.h);
+    if (c->blu[axis]) fill_bluestein<R>(w, c->n[axis], n);
     if (c->tw[axis]) {
         SFD_HIP(hipStreamSynchronize(c->stream));
         SFD_HIP(hipFree(c->tw[axis]));
@@ -448,7 +505,7 @@ template <typename P>
 int split_energy_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt,
                       int64_t nsteps, int order, int mode, double* energies)
 {
-    if (Om != 0.0 || mode != SFD_REAL_TIME) {
+    if (Om != 0.0 || mode != SFD_REAL_TIME || c->blu[0] || c->blu[1] || c->blu[2]) {
         for (int64_t s = 0; s < nsteps; ++s) {
             SFD_TRY(energy_impl<P>(c, psi, V, cD, beta, Om, energies + 5 * s));
             SFD_TRY(split_impl<P>(c, psi, V, cD, beta, Om, dt, 1, order, mode));
@@ -679,7 +736,7 @@ int autotune_impl(sfd_ctx* c, void* scratch, float* times_out)
         if (hipGetLastError() != hipSuccess) rc = fail(SFD_ERR_HIP, "scratch fill failed");
     }
     for (int ax = 0; rc == SFD_OK && ax < c->ndim; ++ax) {
-        const int nv = n_variants<P>(c->n[ax], ax == 0);
+        const int nv = axis_variants<P>(c, ax);
         int best = 0;
         float best_ms = 1e30f;
         for (int v = 0; rc == SFD_OK && v < nv; ++v) {
@@ -1146,7 +1203,7 @@ PrecOps make_ops()
     o.adjoint = &adjoint_impl<P>;
     o.autotune = &autotune_impl<P>;
     o.set_variant = &set_variant<P>;
-    o.n_variants = &n_variants<P>;
+    o.n_variants = &axis_variants<P>;
     o.build_phase_into = &launch_build_phase<P>;
     o.table_entry_bytes = sizeof(cx<typename P::R>);
     o.batch_level_setup = &batch_level_setup<P>;

@@ -156,6 +156,10 @@ struct Workspace {
 struct sfd_ctx {
     int ndim = 0;
     int n[3] = {1, 1, 1};
+    // Bluestein axes (n not a power of two) transform on m = 2^ceil(log2(2n - 1))
+    // points; elsewhere m = n
+    int m[3] = {1, 1, 1};
+    bool blu[3] = {false, false, false};
     double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1}, d[3] = {1, 1, 1};
     long long nsite = 1;
     int prec = SFD_F64;
@@ -377,7 +381,7 @@ struct PrecOps {
                    void* psi_final);
     int (*autotune)(sfd_ctx* c, void* scratch, float* times_out);
     int (*set_variant)(sfd_ctx* c, int axis, int idx);
-    int (*n_variants)(int n, bool contig);
+    int (*n_variants)(const sfd_ctx* c, int axis);
     int (*build_phase_into)(const sfd_ctx* c, hipStream_t stream, void* dev, int axis, double tau, double cD,
                             double Om, int imag, int rows);
     size_t table_entry_bytes;  // bytes of one phase-table entry

@@ -88,6 +88,7 @@ struct SweepArgs {
     int ntx;                // strided flavour: x-tiles per "other" index
     void* save;             // kFlagNL: the state before the nonlinear phase is also stored here
     double* epartial;       // energy round trips: per-workgroup sums [k * nblocks + block], k < 4
+    int nlen;               // Bluestein sweeps: points per line (the transform runs on M >= 2 nlen - 1)
 };
 
 template <typename R, int N1, int N2, int TS, int G>
@@ -869,6 +870,286 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
         double acc1[1] = {nacc};
         block_partials<R, Sh::NT, 1>(acc1, a.partial, gridDim.x, blockIdx.x);
     }
+}
+
+// ------------------------------------------------------------- Bluestein
+// Lines of any length n (n not a power of two) run Bluestein's chirp
+// transform on M = H * N1 * N2 >= 2n - 1 points held in registers (zero
+// beyond n):
+//   X_k = c_k sum_j (x_j c_j) b_{k-j},  c_m = exp(-i pi m^2/n),  b = conj(c),
+// the convolution being an M-point circular one with b wrapped to
+// g[m] = b_m (m < n), g[M - r] = b_r (0 < r < n), done as forward M-point
+// DFT, multiply by B = DFT_M(g)/M, unscaled inverse (L. I. Bluestein, IEEE
+// Trans. Audio Electroacoust. 18(4), 1970).  The inverse DFT is
+// conj(DFT(conj X)).  Tables after the M-point twiddles: chirp[M] (c_p for
+// p < n, zero beyond) and B[M], plain complex in both precisions.
+// A lane holds the line's positions P(j) = tl + H N2 j, tl = H t + h.
+
+// Line geometry of a Bluestein tile: as tile_geom, with n points per
+// contiguous line.
+template <int HN2, int TS, int NL>
+__device__ __forceinline__ TileGeom tile_geom_blu(const SweepArgs& a, long long tile, int L, int tl)
+{
+    TileGeom g;
+    g.row = 0;
+    g.coord_idx = 0;
+    const long long n = a.nlen;
+    if constexpr (TS == 1) {
+        const long long line0 = tile * NL;
+        const long long rem = a.nlines - line0;
+        const int valid = rem < NL ? (int)rem : NL;
+        const int Lc = L < valid ? L : valid - 1;
+        g.active = L < valid;
+        g.ubase = line0 * n;
+        g.lane = (unsigned)(Lc * n + tl);
+        g.jstride = HN2;
+        g.kstride = 1;
+        const long long line = line0 + Lc;
+        if (a.row_mod > 0) g.row = (int)(line % a.row_mod);
+        if (a.coord_mod > 0) g.coord_idx = line % a.coord_mod;
+    } else {
+        const long long o = tile / a.ntx;
+        const int col0 = (int)(tile - o * a.ntx) * NL;
+        const int rem = a.ncols - col0;
+        const int valid = rem < NL ? rem : NL;
+        const int Lc = L < valid ? L : valid - 1;
+        g.active = L < valid;
+        g.ubase = (long long)col0 + o * a.OS;
+        g.lane = (unsigned)Lc + (unsigned)tl * (unsigned)a.S;
+        g.jstride = (long long)HN2 * a.S;
+        g.kstride = a.S;
+        if (a.row_from_col) g.row = col0 + Lc;
+        g.coord_idx = col0 + Lc;
+    }
+    return g;
+}
+
+// Predicated loads and stores of the line's positions below n.
+template <typename S, typename R, int N1, int HN2>
+__device__ __forceinline__ void load_line_blu(cx<R> (&v)[N1], const cx<S>* psi, const TileGeom& g, int tl, int n)
+{
+    const cx<S>* b = psi + g.ubase;
+#pragma unroll
+    for (int j = 0; j < N1; ++j)
+        v[j] = (tl + HN2 * j < n) ? widen<R>(lane_ref(b, j * g.jstride, g.lane)) : cx<R>{R(0), R(0)};
+}
+template <typename S, typename R, int N1, int HN2>
+__device__ __forceinline__ void load_pot_blu(R (&vp)[N1], const S* V, const TileGeom& g, int tl, int n)
+{
+#pragma unroll
+    for (int j = 0; j < N1; ++j) vp[j] = R(0);
+    if (V) {
+        const S* b = V + g.ubase;
+#pragma unroll
+        for (int j = 0; j < N1; ++j)
+            if (tl + HN2 * j < n) vp[j] = R(lane_ref(b, j * g.jstride, g.lane));
+    }
+}
+template <typename S, typename R, int N1, int HN2>
+__device__ __forceinline__ void store_line_blu(const cx<R> (&v)[N1], cx<S>* psi, const TileGeom& g, int tl, int n)
+{
+    if (g.active) {
+        cx<S>* b = psi + g.ubase;
+#pragma unroll
+        for (int j = 0; j < N1; ++j)
+            if (tl + HN2 * j < n) lane_ref(b, j * g.jstride, g.lane) = narrow<S>(v[j]);
+    }
+}
+
+// DFT of the lane's line in place: v holds x at positions P(j) (zero at and
+// beyond n) and gets X_k at position k (zero at and beyond n).  chirp is the
+// chirp table offset by tl, Bt the B table offset by the lane's first
+// spectrum position t + M_sub h.
+template <typename R, int N1, int N2, int TS, int G, int H>
+__device__ __forceinline__ void bluestein_dft(cx<R> (&v)[N1], cx<R>* lds, const cx<R>* twA, const cx<R>* twB,
+                                              const cx<R>* twC, const cx<R>* chirp, const cx<R>* Bt, int c, int g,
+                                              int t, int h)
+{
+    constexpr int HN2 = H * N2;
+    constexpr int RB = N1 / N2;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) v[j] = cmul(v[j], chirp[HN2 * j]);
+    cx<R> w[RB][N2];
+    forward_ab<R, N1, N2, TS, G>(v, w, lds, twA, c, g, t);
+    if constexpr (H == 2) radix2_fwd<R, N1, N2, TS>(w, twC, h);
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) w[r][k2] = cmul(w[r][k2], Bt[N2 * r + N1 * k2]);
+    if constexpr (H == 2) radix2_inv<R, N1, N2, TS>(w, twC, h);
+    inverse_ba<R, N1, N2, TS, G>(w, v, lds, twB, c, g, t);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) v[j] = cmul(v[j], chirp[HN2 * j]);
+}
+
+template <typename R, int N1>
+__device__ __forceinline__ void conj_all(cx<R> (&v)[N1])
+{
+#pragma unroll
+    for (int j = 0; j < N1; ++j) v[j].im = -v[j].im;
+}
+
+// Round trip of a Bluestein tile: DFT, the phase multiplier (indexed by the
+// natural position k, n entries per row), inverse DFT, then the epilogue of
+// round_trip_tile (nonlinear phase, saved state, norm partials).
+template <typename S, typename R, int N1, int N2, int TS, int G, int H, bool IMAG>
+__device__ __forceinline__ void round_trip_blu(cx<R> (&v)[N1], const R (&vp)[N1], const TileGeom& gm,
+                                               const SweepArgs& a, cx<R>* lds, const cx<R>* twA, const cx<R>* twB,
+                                               const cx<R>* twC, const cx<R>* chirp, const cx<R>* Bt,
+                                               const cx<R>* phase, int c, int g, int t, int h, int tl,
+                                               double& norm_acc)
+{
+    constexpr int HN2 = H * N2;
+    const int n = a.nlen;
+    const int flags = a.flags;
+    if (flags & kFlagScale) {
+        const R sc = R(*a.scale);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) v[j] = cscale(v[j], sc);
+    }
+    bluestein_dft<R, N1, N2, TS, G, H>(v, lds, twA, twB, twC, chirp, Bt, c, g, t, h);
+    const R inv_n = R(1) / R(n);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+        const int p = tl + HN2 * j;
+        const cx<R> d = phase[p < n ? p : 0];
+        const cx<R> x = v[j];
+        if constexpr (sizeof(R) == 4) {
+            // FP32 tables as in round_trip_tile: lifting coefficients of half
+            // the phase (real time), (m - 1)/n (imaginary time); 1/n rounded
+            if constexpr (IMAG) v[j] = {fmaf(x.re, d.re, x.re * inv_n), fmaf(x.im, d.re, x.im * inv_n)};
+            else v[j] = lift(lift({x.re * inv_n, x.im * inv_n}, d.re, d.im), d.re, d.im);
+        } else {
+            v[j] = cmul(x, d);
+        }
+    }
+    conj_all<R, N1>(v);
+    bluestein_dft<R, N1, N2, TS, G, H>(v, lds, twA, twB, twC, chirp, Bt, c, g, t, h);
+    conj_all<R, N1>(v);
+    if (flags & kFlagNL) {
+        if (a.save) store_line_blu<S, R, N1, HN2>(v, static_cast<cx<S>*>(a.save), gm, tl, n);
+        const R beta = R(a.beta), dt = R(a.dt);
+        const bool imag = sizeof(R) == 4 ? IMAG : (flags & kFlagImag) != 0;
+#pragma unroll
+        for (int j = 0; j < N1; ++j) nl_phase<R>(v[j], vp[j], beta, dt, imag);
+    }
+    if (flags & kFlagNorm) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            const double re = v[j].re, im = v[j].im;
+            s += re * re + im * im;
+        }
+        if (gm.active) norm_acc += s;
+    }
+}
+
+// One launch over all tiles of an axis whose lines have n = a.nlen points
+// (not a power of two), by Bluestein transforms on M = H N1 N2 points.
+// Modes: round trips (single and [d N d]), forward, inverse, energy sums.
+template <typename S, typename R, int N1, int N2, int TS, int G, int MODE, int H = 1>
+__global__ __launch_bounds__(TS * G * N2 * H) void k_sweep_blu(const SweepArgs a)
+{
+    using Sh = SweepShape<R, N1, N2, TS, H * G>;
+    constexpr int M = Sh::N;  // sub-transform length; the transform has H * M points
+    constexpr int RB = Sh::RB;
+    constexpr int NL = TS * G;
+    constexpr int HN2 = H * N2;
+    constexpr int NTW = kTwiddles<M, H>;
+    __shared__ cx<R> lds[Sh::LDS];
+    const int tid = threadIdx.x;
+    const int c = tid % TS;
+    const int tl = (tid / TS) % HN2;
+    const int t = tl / H;
+    const int h = tl % H;
+    const int g = tid / (TS * HN2);
+    const int gh = H * g + h;
+    const int L = c + TS * g;
+    const long long tile = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+    const TileGeom gm = tile_geom_blu<HN2, TS, NL>(a, tile, L, tl);
+    const int n = a.nlen;
+    const cx<R>* tw = static_cast<const cx<R>*>(a.tw);
+    const cx<R>* twA = tw + t;
+    const cx<R>* twB = tw + M + t;
+    const cx<R>* twC = tw + 2 * M + t;
+    const cx<R>* chirp = tw + NTW + tl;
+    const cx<R>* Bt = tw + NTW + H * M + t + M * h;
+    cx<S>* __restrict__ psi = static_cast<cx<S>*>(a.psi);
+    const long long nblocks = (long long)gridDim.x * gridDim.y;
+    cx<R> v[N1];
+    R vp[N1];
+    if constexpr (is_round_trip(MODE) || rt_twice(MODE)) {
+        static_assert(!rt_energy(MODE), "no energy round trips on Bluestein lines");
+        load_line_blu<S, R, N1, HN2>(v, psi, gm, tl, n);
+        if (a.flags & kFlagNL) load_pot_blu<S, R, N1, HN2>(vp, static_cast<const S*>(a.V), gm, tl, n);
+        else
+#pragma unroll
+            for (int j = 0; j < N1; ++j) vp[j] = R(0);
+        const cx<R>* ph = static_cast<const cx<R>*>(a.phase) + (long long)gm.row * a.phase_row_stride;
+        double nacc = 0.0;
+        if constexpr (rt_twice(MODE)) {
+            SweepArgs a1 = a;
+            a1.flags = a.flags & ~kFlagNorm;
+            SweepArgs a2 = a;
+            a2.flags = a.flags & ~(kFlagNL | kFlagScale);
+            a2.save = nullptr;
+            round_trip_blu<S, R, N1, N2, TS, H * G, H, rt_imag(MODE)>(v, vp, gm, a1, lds, twA, twB, twC, chirp, Bt,
+                                                                     ph, c, gh, t, h, tl, nacc);
+            round_trip_blu<S, R, N1, N2, TS, H * G, H, rt_imag(MODE)>(v, vp, gm, a2, lds, twA, twB, twC, chirp, Bt,
+                                                                     ph, c, gh, t, h, tl, nacc);
+        } else {
+            round_trip_blu<S, R, N1, N2, TS, H * G, H, rt_imag(MODE)>(v, vp, gm, a, lds, twA, twB, twC, chirp, Bt,
+                                                                     ph, c, gh, t, h, tl, nacc);
+        }
+        if (a.flags & kFlagNorm) {
+            double acc1[1] = {nacc};
+            block_partials<R, Sh::NT, 1>(acc1, a.partial, nblocks, tile);
+        }
+        store_line_blu<S, R, N1, HN2>(v, psi, gm, tl, n);
+    } else if constexpr (MODE == kForward) {
+        load_line_blu<S, R, N1, HN2>(v, psi, gm, tl, n);
+        bluestein_dft<R, N1, N2, TS, H * G, H>(v, lds, twA, twB, twC, chirp, Bt, c, gh, t, h);
+        store_line_blu<S, R, N1, HN2>(v, static_cast<cx<S>*>(a.out), gm, tl, n);
+    } else if constexpr (MODE == kInverse) {
+        load_line_blu<S, R, N1, HN2>(v, psi, gm, tl, n);
+        conj_all<R, N1>(v);
+        bluestein_dft<R, N1, N2, TS, H * G, H>(v, lds, twA, twB, twC, chirp, Bt, c, gh, t, h);
+        const R inv = R(1) / R(n);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) v[j] = {v[j].re * inv, -v[j].im * inv};
+        store_line_blu<S, R, N1, HN2>(v, static_cast<cx<S>*>(a.out), gm, tl, n);
+    } else if constexpr (MODE == kEnergy) {
+        double acc[5] = {0, 0, 0, 0, 0};
+        load_line_blu<S, R, N1, HN2>(v, psi, gm, tl, n);
+        if (a.flags & kFlagPointwise) {
+            load_pot_blu<S, R, N1, HN2>(vp, static_cast<const S*>(a.V), gm, tl, n);
+#pragma unroll
+            for (int j = 0; j < N1; ++j) {
+                const double re = v[j].re, im = v[j].im;
+                const double a2 = re * re + im * im;
+                acc[2] += a2;
+                acc[3] += (double)vp[j] * a2;
+                acc[4] += a2 * a2;
+            }
+        }
+        bluestein_dft<R, N1, N2, TS, H * G, H>(v, lds, twA, twB, twC, chirp, Bt, c, gh, t, h);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            const int p = tl + HN2 * j;
+            const double x = p < n ? a.xi[p] : 0.0;
+            const double re = v[j].re, im = v[j].im;
+            const double pw = re * re + im * im;
+            acc[0] += x * x * pw;
+            acc[1] += x * pw;
+        }
+        if (a.coord) acc[1] *= a.coord[gm.coord_idx];
+        if (!gm.active) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) acc[k] = 0.0;
+        }
+        block_partials<R, Sh::NT, 5>(acc, a.partial, nblocks, tile);
+    }
+    (void)RB;
 }
 
 }  // namespace sfd

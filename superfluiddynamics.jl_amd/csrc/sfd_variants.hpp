@@ -80,6 +80,7 @@ SFD_VARIANTS(PF32, 256, (VL<V<16, 16, 1, 16>, V<16, 16, 1, 8>, V<32, 8, 1, 16>, 
 SFD_VARIANTS(PF32, 512, (VL<V<32, 16, 1, 8>, V<32, 16, 1, 4>, V<16, 16, 1, 8, 0, 1, 2>, V<16, 16, 1, 4, 0, 1, 2>>),
              (VL<V<32, 16, 16, 1>, V<32, 16, 8, 1>, V<16, 16, 16, 1, 0, 1, 2>, V<16, 16, 8, 1, 0, 1, 2>>))
 SFD_VARIANTS(PF32, 1024, (VL<V<32, 32, 1, 4>, V<32, 32, 1, 2>>), (VL<V<32, 32, 8, 1>, V<32, 32, 4, 1>>))
+SFD_VARIANTS(PF32, 2048, (VL<V<32, 32, 1, 2, 0, 1, 2>>), (VL<V<32, 32, 4, 1, 0, 1, 2>>))
 
 // ---------------------------------------------------------------- FP64
 SFD_VARIANTS(PF64, 2, (VL<V<2, 1, 1, 256>>), (VL<V<2, 1, 16, 4>>))
@@ -98,6 +99,7 @@ SFD_VARIANTS(PF64, 512,
              (VL<V<16, 16, 8, 1, 0, 1, 2>, V<32, 16, 8, 1>, V<32, 16, 4, 1>, V<16, 16, 16, 1, 0, 1, 2>,
                  V<16, 16, 4, 1, 0, 1, 2, 1>, V<16, 16, 8, 1, 0, 1, 2, 1>>))
 SFD_VARIANTS(PF64, 1024, (VL<V<32, 32, 1, 2>>), (VL<V<32, 32, 4, 1>, V<32, 32, 2, 1>>))
+SFD_VARIANTS(PF64, 2048, (VL<V<32, 32, 1, 1, 0, 1, 2>>), (VL<V<32, 32, 2, 1, 0, 1, 2>>))
 
 #undef SFD_VARIANTS
 
@@ -214,7 +216,7 @@ bool info_variant_n(bool contig, int idx, VariantInfo* out)
     return Dispatch<P, 0, typename Variants<P, N>::strided>::info(idx, out);
 }
 
-#define SFD_FOR_EACH_N(M) M(2) M(4) M(8) M(16) M(32) M(64) M(128) M(256) M(512) M(1024)
+#define SFD_FOR_EACH_N(M) M(2) M(4) M(8) M(16) M(32) M(64) M(128) M(256) M(512) M(1024) M(2048)
 
 template <typename P>
 int n_variants(int n, bool contig)
@@ -280,6 +282,43 @@ bool launch_variant(int n, bool contig, int idx, dim3 grid, dim3 block, hipStrea
 #define SFD_CASE(N_) \
     if (n == N_) return launch_variant_n<P, MODE, N_>(contig, idx, grid, block, s, a, persistent);
     SFD_FOR_EACH_N(SFD_CASE)
+#undef SFD_CASE
+    return false;
+}
+
+// Bluestein sweeps (lines of n points, n not a power of two) run on the
+// first variant of the M-point lists, M = 2^ceil(log2(2n - 1)) in [8, 2048].
+template <typename L>
+struct FirstOf;
+template <typename V0, typename... Vs>
+struct FirstOf<VL<V0, Vs...>> {
+    using type = V0;
+};
+
+template <typename P, int MODE, int N>
+bool launch_blu_n(bool contig, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a)
+{
+    using S = typename P::S;
+    using R = typename P::R;
+    using VC = typename FirstOf<typename Variants<P, N>::contig>::type;
+    using VS = typename FirstOf<typename Variants<P, N>::strided>::type;
+    if (contig) {
+        if constexpr (rt_twice(MODE)) return false;  // the double round trip runs on a strided axis only
+        else hipLaunchKernelGGL((k_sweep_blu<S, R, VC::N1, VC::N2, VC::TS, VC::G, MODE, VC::H>), grid, block, 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_sweep_blu<S, R, VS::N1, VS::N2, VS::TS, VS::G, MODE, VS::H>), grid, block, 0, s, a);
+    }
+    return true;
+}
+
+#define SFD_FOR_EACH_M(M) M(8) M(16) M(32) M(64) M(128) M(256) M(512) M(1024) M(2048)
+
+template <typename P, int MODE>
+bool launch_blu(int m, bool contig, dim3 grid, dim3 block, hipStream_t s, const SweepArgs& a)
+{
+#define SFD_CASE(N_) \
+    if (m == N_) return launch_blu_n<P, MODE, N_>(contig, grid, block, s, a);
+    SFD_FOR_EACH_M(SFD_CASE)
 #undef SFD_CASE
     return false;
 }

@@ -57,8 +57,8 @@ def _create(lib, ndim, n, lo, hi, prec):
 @pytest.mark.parametrize("ndim,n,lo,hi,prec,code", [
     (4, (8, 8, 8), (0, 0, 0), (1, 1, 1), 2, -2),      # ndim out of range
     (1, (8, 8, 8), (0, 0, 0), (1, 1, 1), 2, -2),
-    (2, (48, 8, 1), (0, 0, 0), (1, 1, 1), 2, -2),     # not a power of two
-    (3, (8, 8, 2048), (0, 0, 0), (1, 1, 1), 1, -2),   # above 1024
+    (2, (1025, 8, 1), (0, 0, 0), (1, 1, 1), 2, -2),   # not a power of two, above 1024
+    (3, (8, 8, 4096), (0, 0, 0), (1, 1, 1), 1, -2),   # power of two above 2048
     (3, (8, 8, 1), (0, 0, 0), (1, 1, 1), 1, -2),      # below 2
     (2, (8, 8, 1), (0, 1, 0), (1, 1, 1), 2, -3),      # empty axis (Grids.jl:75)
     (2, (8, 8, 1), (0, 0, 0), (1, 1, 1), 7, -3),      # bad precision
@@ -96,7 +96,7 @@ def test_error_mapping_to_exceptions():
     import sfdhip
     from sfdhip import _lib as L
     lib = sfdhip.load()
-    rc, _ = _create(lib, 2, (48, 8, 1), (0, 0, 0), (1, 1, 1), 2)
+    rc, _ = _create(lib, 2, (1025, 8, 1), (0, 0, 0), (1, 1, 1), 2)
     with pytest.raises(sfdhip.ShapeError):
         L.check(rc)
     rc, _ = _create(lib, 2, (8, 8, 1), (0, 0, 0), (1, 1, 1), 9)

@@ -139,8 +139,12 @@ def test_error_paths(gpu):
     rc = lib.sfd_split_step(c.h, ctypes.c_void_p(f.ϕ.data_ptr()), None, -0.5, 1.0, 0.0, 1e-3, 1, 3, 0)
     assert rc == -3
     assert b"order" in lib.sfd_last_error()
-    with pytest.raises(gpu.ShapeError):
-        gpu.Field(gpu.Grid((48, 32), ((-1, 1), (-1, 1)))).ctx()
+    # any n in [3, 1024] and powers of two up to 2048 are accepted (Bluestein
+    # lines for the others); larger lines are refused
+    gpu.Field(gpu.Grid((48, 32), ((-1, 1), (-1, 1)))).ctx()
+    for bad in ((1025, 8), (4096, 2), (8, 1)):
+        with pytest.raises(gpu.ShapeError):
+            gpu.Field(gpu.Grid(bad, ((-1, 1), (-1, 1)))).ctx()
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
