@@ -64,7 +64,7 @@ PRECISIONS = {"f32": ("float32", None), "f32x": ("float32", "float64"), "f64": (
 SWEEP_SLOTS = ("x round trip", "y round trip", "z round trip", "double round trip [d N d]", "other sweeps")
 
 
-def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, reps, autotune=True):
+def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, reps, launches, settle_ms=200.0, autotune=True):
     """One precision: autotune, warm up, then `reps` timed runs of `steps`
     Split2 steps (HIP events on the context's stream, which is torch's
     current stream); ms/step is their median.  A further run of `steps`
@@ -83,6 +83,15 @@ def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, reps, autotune=True):
     model = sfd.NumModelSplit2(f, p, mp["dt"], steps, steps)
     model.steps(max(warmup, 1))
     torch.cuda.synchronize()
+    # settle: further untimed steps until settle_ms of GPU work have run, so
+    # that the clocks, lowered while the host prepared the field, are back
+    # up before the timed runs
+    settle_steps = 0
+    t_settle = time.perf_counter()
+    while (time.perf_counter() - t_settle) * 1e3 < settle_ms:
+        model.steps(10)
+        torch.cuda.synchronize()
+        settle_steps += 10
     runs = []
     for _ in range(reps):
         e0 = torch.cuda.Event(enable_timing=True)
@@ -110,10 +119,30 @@ def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, reps, autotune=True):
         if cnt[k]:
             sweeps[name] = {"launches_per_step": cnt[k] / steps, "avg_ms": tot[k] / cnt[k],
                             "ms_per_step": tot[k] / steps}
+    # The one-axis round trips of the step (x, y: the same kernels, variants
+    # and flags as in the step) launched back to back on the context's
+    # stream: HIP events around `launches` launches, median of 3 batches.
+    # The in-step events above add a few microseconds to every launch; these
+    # do not.
+    back_to_back = {}
+    psi = ctypes.c_void_p(f.ϕ.data_ptr())
+    for ax, name in ((0, "x round trip"), (1, "y round trip")):
+        reps = []
+        for _ in range(3):
+            k0 = torch.cuda.Event(enable_timing=True)
+            k1 = torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            k0.record()
+            for _ in range(launches):
+                lib.sfd_lap_rot_axis(ctx.h, psi, ax, mp["coeffD"], 0.0, mp["dt"] * 0.5, 0)
+            k1.record()
+            torch.cuda.synchronize()
+            reps.append(k0.elapsed_time(k1) / launches)
+        back_to_back[name] = sorted(reps)[1]
     del model, f
     torch.cuda.empty_cache()
     return dict(ms=ms_step, runs_ms=runs, state=state, norm=nrm, variants=variants, tune=tune, solve=solve,
-                sweeps=sweeps)
+                sweeps=sweeps, back_to_back=back_to_back, settle_steps=settle_steps)
 
 
 def solve_loop(sfd, torch, model, k):
@@ -272,6 +301,10 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--reps", type=int, default=5, help="timed runs of --steps steps; ms/step is their median")
+    ap.add_argument("--settle-ms", type=float, default=200.0,
+                    help="untimed steps after the warm-up until this much GPU time has passed (clocks settle)")
+    ap.add_argument("--kernel-launches", type=int, default=40,
+                    help="back-to-back launches timed per batch for the dominant kernel's duration")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the 1000-step FP32/F32X-vs-FP64 accuracy run at the stable dt")
@@ -308,7 +341,8 @@ def main(argv=None):
     res = {}
     states = {}
     for prec in ("f32", "f32x", "f64"):
-        r = gpu_leg(sfd, torch, psi0, mp, prec, args.steps, args.warmup, args.reps, not args.no_autotune)
+        r = gpu_leg(sfd, torch, psi0, mp, prec, args.steps, args.warmup, args.reps, args.kernel_launches,
+                    args.settle_ms, not args.no_autotune)
         states[prec] = r.pop("state")
         res[prec] = r
         solve = r["solve"]
@@ -341,12 +375,15 @@ def main(argv=None):
                          variants=r["variants"])
 
     # Dominant kernel: the sweep kernel with the largest share of the FP32
-    # step's time, timed inside the step (one read and one write of the
-    # complex FP32 field per launch; V = 0, so no potential bytes).
+    # step's time (in-step timer), one read and one write of the complex FP32
+    # field per launch (V = 0: no potential bytes).  Its average duration is
+    # the back-to-back figure when it is a one-axis round trip (x or y), else
+    # the in-step one.
     k32 = res["f32"]["sweeps"]
     dom = max((k for k in k32 if k != "other sweeps"), key=lambda k: k32[k]["ms_per_step"])
+    dom_ms = res["f32"]["back_to_back"].get(dom, k32[dom]["avg_ms"])
     launch_bytes = 2 * 8 * sites
-    achieved = launch_bytes / (k32[dom]["avg_ms"] * 1e-3)
+    achieved = launch_bytes / (dom_ms * 1e-3)
     traffic = load_traffic(dom)
     dom_kernel = kernel_name(res["f32"]["variants"], SWEEP_SLOTS.index(dom))
 
@@ -403,7 +440,11 @@ def main(argv=None):
                                   "priced at the 6-pass 96 B of the reference's structure it would read "
                                   f"{out['f32']['step_hbm_frac_6_pass_bytes']:.3f}",
         "f32_kernels": out["f32"]["kernels"],
+        "f32_back_to_back_ms": res["f32"]["back_to_back"],
         "f32_runs_ms": out["f32"]["runs_ms"],
+        "settle": {"ms": args.settle_ms, "f32_steps": res["f32"]["settle_steps"],
+                   "note": "untimed steps after the warm-up steps, until the clocks lowered during host-side "
+                           "set-up are back up; then --reps timed runs of exactly --steps steps each"},
         "f32_variants": out["f32"]["variants"],
         "f32_vs_f64_rel_l2": err32,
         "f32x_vs_f64_rel_l2": err32x,
@@ -422,7 +463,11 @@ def main(argv=None):
                      "kernel": f"{dom_kernel}: {dom}, FP32, {n}³ (the step's largest share of time, "
                                f"{k32[dom]['ms_per_step'] * 1e3:.1f} us of {out['f32']['ms_per_step'] * 1e3:.1f} "
                                f"us per step)",
-                     "avg_launch_ms": k32[dom]["avg_ms"], "bytes_per_launch": launch_bytes},
+                     "avg_launch_ms": dom_ms, "avg_launch_ms_in_step": k32[dom]["avg_ms"],
+                     "bytes_per_launch": launch_bytes,
+                     "timing": "avg_launch_ms: HIP events around back-to-back launches on the context's stream; "
+                               "avg_launch_ms_in_step: events around each launch inside the timed steps "
+                               "(adds a few us per launch)"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
