@@ -725,7 +725,11 @@ int sfd_ctx_destroy(sfd_ctx* ctx)
 int sfd_ctx_set_stream(sfd_ctx* ctx, void* stream)
 {
     SFD_TRY(check_ctx(ctx));
-    ctx->stream = static_cast<hipStream_t>(stream);
+    // Tables, partials and workspaces are reused in stream order: work still
+    // queued on the old stream must end before the new stream touches them.
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (s != ctx->stream) SFD_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stream = s;
     return SFD_OK;
 }
 

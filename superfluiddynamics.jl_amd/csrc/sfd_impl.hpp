@@ -155,7 +155,10 @@ int launch_build_phase(const sfd_ctx* c, hipStream_t stream, void* dev, int axis
 
 // Phase table of (axis, tau, coeffD, Omega, mode) from the context's cache
 // (least recently used first out, at most 32 entries unless pinned), built
-// on the context's stream on a miss.
+// on the context's stream on a miss.  A full cache rebuilds its least
+// recently used entry in place: the sweeps that read the old table were
+// enqueued earlier on the same stream, so the build kernel runs after them
+// and no host synchronisation or allocation is needed.
 template <typename P>
 int build_phase(sfd_ctx* c, int axis, double tau, double cD, double Om, int imag, void** out, int* rows_out)
 {
@@ -173,21 +176,27 @@ int build_phase(sfd_ctx* c, int axis, double tau, double cD, double Om, int imag
     }
     const int rows = phase_rows(c, axis, Om);
     const size_t bytes = (size_t)rows * c->n[axis] * sizeof(cx<R>);
-    if (c->phases.size() >= 32 && !c->pin_phases) {  // a sweep may still read an evicted table
-        SFD_HIP(hipStreamSynchronize(c->stream));
-        while (c->phases.size() >= 32) {
-            (void)hipFree(c->phases.front().dev);
-            c->phases.erase(c->phases.begin());
+    void* dev = nullptr;
+    size_t cap = bytes;
+    if (c->phases.size() >= 32 && !c->pin_phases) {
+        PhaseEntry lru = c->phases.front();
+        c->phases.erase(c->phases.begin());
+        if (lru.bytes >= bytes) {
+            dev = lru.dev;
+            cap = lru.bytes;
+        } else {  // a larger table (rotation rows): the old buffer may still be read
+            SFD_HIP(hipStreamSynchronize(c->stream));
+            (void)hipFree(lru.dev);
         }
     }
-    void* dev = nullptr;
-    SFD_HIP(hipMalloc(&dev, bytes));
+    if (!dev) SFD_HIP(hipMalloc(&dev, bytes));
     int rc = launch_build_phase<P>(c, c->stream, dev, axis, tau, cD, Om, imag, rows);
     if (rc != SFD_OK) {
+        SFD_HIP(hipStreamSynchronize(c->stream));
         (void)hipFree(dev);
         return rc;
     }
-    c->phases.push_back({axis, tau, cD, Om, imag, dev, rows});
+    c->phases.push_back({axis, tau, cD, Om, imag, dev, rows, cap});
     *out = dev;
     *rows_out = rows;
     return SFD_OK;

@@ -131,6 +131,7 @@ struct PhaseEntry {
     int imag;
     void* dev;
     int rows;
+    size_t bytes;  // capacity of dev
 };
 
 // Launch geometry of one sweep.

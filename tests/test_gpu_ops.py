@@ -189,3 +189,26 @@ def test_lap_rot_and_nl_compose_to_split2(gpu, orc):
     gpu.solveNL(m, 3e-3)
     gpu.solveLapRot(m, 1.5e-3)
     assert rel_l2(f.to_numpy(), ref) < 1e-12
+
+
+def test_dt_schedule_cycles_the_phase_cache(gpu, orc):
+    """A Δt schedule of 90 distinct steps, with Ω switched on every third
+    step, on one context: the phase-table cache (32 entries) fills and
+    rebuilds its least recently used tables in place on the stream, and
+    grows a table when a rotating step needs more rows.  Every step is
+    checked against the oracle run of the same schedule (FP64, 1e-12)."""
+    shape, bounds = (64, 32), ((-4, 4), (-3, 3))
+    psi = smooth_state(shape, bounds, seed=21)
+    V = random_potential(shape, bounds)
+    f = _field(gpu, psi, bounds)
+    pot = gpu.PotentialArray(f, V)
+    ref = psi.copy()
+    for k in range(90):
+        dt = 1e-3 * (1.0 + k / 97.0)
+        Om = 0.3 if k % 3 == 0 else 0.0
+        p = gpu.GrossPitaevskiiParameters(pot=pot, coeffΔ=-0.5, β=4.0, Ω=Om)
+        gpu.NumModelSplit2(f, p, dt, 1, 1).steps(1)
+        orc.split_step(ref, bounds, V=V, coeffD=-0.5, beta=4.0, Omega=Om, dt=dt, nsteps=1, order=2)
+        if k % 15 == 14:
+            assert rel_l2(f.to_numpy(), ref) < 1e-12, k
+    assert rel_l2(f.to_numpy(), ref) < 1e-12
