@@ -83,6 +83,8 @@ def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, reps, launches, settle_ms
     model = sfd.NumModelSplit2(f, p, mp["dt"], steps, steps)
     model.steps(max(warmup, 1))
     torch.cuda.synchronize()
+    # the precision comparison takes every leg's state after the same steps
+    state = f.to_numpy()
     # settle: further untimed steps until settle_ms of GPU work have run, so
     # that the clocks, lowered while the host prepared the field, are back
     # up before the timed runs
@@ -103,7 +105,6 @@ def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, reps, launches, settle_ms
         torch.cuda.synchronize()
         runs.append(e0.elapsed_time(e1) / steps)
     ms_step = float(np.median(runs))
-    state = f.to_numpy()
     nrm = sfd.norm(f)
     solve = solve_loop(sfd, torch, model, min(steps, 50))
     lib, ctx = sfd.load(), f.ctx()
@@ -448,7 +449,7 @@ def main(argv=None):
         "f32_variants": out["f32"]["variants"],
         "f32_vs_f64_rel_l2": err32,
         "f32x_vs_f64_rel_l2": err32x,
-        "f32_vs_f64_steps": args.warmup + args.steps,
+        "f32_vs_f64_steps": max(args.warmup, 1),
         "f32_vs_f64_note": "at the metric dt the unfiltered split is past its stability limit "
                            "(D |coeffD| kmax^2 dt = 12.3 > pi): any two precisions, or FP64 runs one rounding "
                            "apart, decorrelate to ~sqrt(2); accuracy_stable_dt holds the comparison at dt = 1e-4",

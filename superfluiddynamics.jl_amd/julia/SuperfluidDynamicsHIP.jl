@@ -27,6 +27,7 @@ const libsfd = normpath(joinpath(@__DIR__, "..", "sfdhip", "libsfdhip.so"))
 const SFD_F32, SFD_F64, SFD_F32X = Cint(1), Cint(2), Cint(3)
 const SFD_REAL_TIME, SFD_IMAG_TIME = Cint(0), Cint(1)
 const SFD_FORWARD, SFD_BACKWARD = Cint(-1), Cint(1)
+const SFD_BATCH_GRAPH, SFD_BATCH_ESCALATE = Cint(1), Cint(2)
 
 # Arithmetic of Float32 fields: SFD_F32 (FP32, lifting rotations, FP64
 # reductions) or SFD_F32X (FP32 storage, FP64 arithmetic).  Set before the
@@ -173,30 +174,41 @@ function autotune!(f::HipField)
     return reshape(times, 8, 3)
 end
 
-# Batched 2D imaginary-time ground states of uneven sizes (sfd_batch_*):
-# advances every field until ||ψ_{n+1} - ψ_n|| < tol or max_steps.
+# Batched imaginary-time ground states of uneven sizes, 2D and 3D, with
+# rotation (sfd_batch_*): advances every field until ||ψ_{n+1} - ψ_n|| < tol
+# or max_steps.  With escalate (Float32 fields) each instance runs in FP32
+# down to its rounding floor and finishes in FP64, so tol may be 1e-9.
+# Returns (steps, ||Δψ||, finished_in_fp64) per field.
 function batch_ground_states!(fields::Vector{<:HipField}, params::Vector{<:GrossPitaevskiiParameters}, Δt;
-                              max_steps=20000, tol=1e-9, check_every=100, use_graph=true, tune=true)
+                              max_steps=20000, tol=1e-9, check_every=100, use_graph=true, tune=true,
+                              escalate=true)
     ni = length(fields)
-    n = Int64[v for f in fields for v in f.g.n]
-    box = Cdouble[v for f in fields for a in 1:2 for v in (f.g.min[a], f.g.max[a])]
+    nd = Cint[length(f.g.n) for f in fields]
+    n = Int64[a <= length(f.g.n) ? f.g.n[a] : 1 for f in fields for a in 1:3]
+    box = Cdouble[v for f in fields for a in 1:3
+                  for v in (a <= length(f.g.n) ? (f.g.min[a], f.g.max[a]) : (0.0, 0.0))]
     FT = eltype(real(first(fields).ϕ))
     prec = FT === Float32 ? FLOAT32_MODE[] : SFD_F64
     h = Ref{Ptr{Cvoid}}(C_NULL)
-    check(ccall((:sfd_batch_create, libsfd), Cint, (Ptr{Ptr{Cvoid}}, Cint, Ptr{Int64}, Ptr{Cdouble}, Cint, Ptr{Cvoid}),
-                h, ni, n, box, prec, stream().stream))
+    check(ccall((:sfd_batch_create_nd, libsfd), Cint,
+                (Ptr{Ptr{Cvoid}}, Cint, Ptr{Cint}, Ptr{Int64}, Ptr{Cdouble}, Cint, Ptr{Cvoid}),
+                h, ni, nd, n, box, prec, stream().stream))
     try
         tune && check(ccall((:sfd_batch_autotune, libsfd), Cint, (Ptr{Cvoid}, Ptr{Cfloat}), h[], C_NULL))
         psi = Ptr{Cvoid}[devptr(f.ϕ) for f in fields]
         V = Ptr{Cvoid}[vptr(p) for p in params]
         β = Cdouble[p.β for p in params]
+        Ω = Cdouble[p.Ω for p in params]
+        flags = Cint((use_graph ? SFD_BATCH_GRAPH : 0) | (escalate ? SFD_BATCH_ESCALATE : 0))
         steps = zeros(Int64, ni)
         dpsi = zeros(Cdouble, ni)
-        check(ccall((:sfd_batch_imag_run, libsfd), Cint,
-                    (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Cdouble}, Cdouble, Cdouble, Int64, Cdouble,
-                     Cint, Cint, Ptr{Int64}, Ptr{Cdouble}),
-                    h[], psi, V, β, first(params).coeffΔ, Δt, max_steps, tol, check_every, use_graph, steps, dpsi))
-        return steps, dpsi
+        level = zeros(Cint, ni)
+        check(ccall((:sfd_batch_imag_solve, libsfd), Cint,
+                    (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Cdouble}, Ptr{Cdouble}, Cdouble, Cdouble, Int64,
+                     Cdouble, Cint, Cint, Ptr{Int64}, Ptr{Cdouble}, Ptr{Ptr{Cvoid}}, Ptr{Cint}),
+                    h[], psi, V, β, Ω, first(params).coeffΔ, Δt, max_steps, tol, check_every, flags, steps, dpsi,
+                    C_NULL, level))
+        return steps, dpsi, level .== 1
     finally
         ccall((:sfd_batch_destroy, libsfd), Cint, (Ptr{Cvoid},), h[])
     end

@@ -195,9 +195,9 @@ def test_batch_cfg4_fp32_escalates_to_fp64_and_converges(gpu, prec):
     every instance runs in FP32 to its rounding floor, finishes in FP64 and
     meets ||Δψ|| < 1e-9 well within 20000 steps.  Its final state lies as
     close to the FP64 fixed point (an FP64 batch run to 1e-12) as the FP64
-    batch stopped at 1e-9 does, and its energy matches to 1e-10 relative
-    (the energy is stationary at the ground state; μ is not and is compared
-    at the state's own accuracy, 1e-7).  Three sampled instances also equal
+    batch stopped at 1e-9 does, and every instance's energy matches the FP64
+    batch's to 1e-10 relative (the energy is stationary at the ground state;
+    μ is not and is compared at the states' own accuracy, 1e-7).  Three sampled instances also equal
     the single-field FP64 imaginary-time fixed point (sfd_split_step), which
     the oracle pins in test_gpu_ground_state."""
     import time
@@ -232,6 +232,10 @@ def test_batch_cfg4_fp32_escalates_to_fp64_and_converges(gpu, prec):
         e_64.append(float((ref64[i] - fixed).abs().pow(2).sum().sqrt() / nrm))
         # the returned FP32 field is the FP64 state rounded once
         assert rel_l2(fields[i].to_numpy(), out64[i].cpu().numpy()) < 1e-7
+        # every instance's energy and chemical potential against the FP64 batch's
+        Ee, E6 = _energy64(gpu, pr, out64[i]), _energy64(gpu, pr, ref64[i])
+        assert abs(Ee[3] - E6[3]) <= 1e-10 * abs(E6[3]), (i, Ee[3], E6[3])
+        assert abs(Ee[4] - E6[4]) <= 1e-7 * abs(E6[4]), (i, Ee[4], E6[4])
     e_esc, e_64 = np.array(e_esc), np.array(e_64)
     print(f"cfg4 {prec}: distance to the FP64 fixed point: escalated median {np.median(e_esc):.2e} "
           f"max {e_esc.max():.2e}; FP64 batch median {np.median(e_64):.2e} max {e_64.max():.2e}")
