@@ -540,8 +540,11 @@ __device__ __forceinline__ void stage_tables(cx<R>* tabs, const SweepArgs& a, in
 // must leave room for (1 = no constraint).  H = 2: lines of 2*N1*N2 points,
 // two lanes per sub-transform lane (see radix2_fwd).  TL: the round trips
 // read the twiddle tables from LDS, and the phase table too when it has one
-// row (no rotation term on the axis).
-template <typename S, typename R, int N1, int N2, int TS, int G, int MODE, int W = 1, int H = 1, bool TL = false>
+// row (no rotation term on the axis).  ZV: double round trips without a
+// potential (V = 0, a null pointer), so that the nonlinear phase holds no
+// potential registers (N1 fewer VGPRs).
+template <typename S, typename R, int N1, int N2, int TS, int G, int MODE, int W = 1, int H = 1, bool TL = false,
+          bool ZV = false>
 __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
 {
     using Sh = SweepShape<R, N1, N2, TS, H * G>;  // LDS lines: one per half-line
@@ -627,7 +630,12 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
         // of the loaded state (spectral sum only: this is never the x axis).
         constexpr bool EN = rt_energy(MODE);
         load_line<S, R, N1>(v, psi, gm);
-        if (flags & kFlagNL) load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
+        if constexpr (ZV) {
+#pragma unroll
+            for (int j = 0; j < N1; ++j) vp[j] = R(0);
+        } else if (flags & kFlagNL) {
+            load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
+        }
         double nacc = 0.0;
         double eacc[4] = {0, 0, 0, 0};
         SweepArgs a1 = a;

@@ -77,8 +77,11 @@ SFD_VARIANTS(PF32, 128, (VL<V<16, 8, 1, 32>, V<16, 8, 1, 16>, V<32, 4, 1, 32>, V
              (VL<V<16, 8, 16, 1>, V<16, 8, 32, 1>, V<32, 4, 32, 1>, V<16, 8, 16, 1, 1>>))
 SFD_VARIANTS(PF32, 256, (VL<V<16, 16, 1, 16>, V<16, 16, 1, 8>, V<32, 8, 1, 16>, V<16, 16, 1, 16, 1>, V<16, 16, 1, 8, 1>>),
              (VL<V<16, 16, 16, 1>, V<16, 16, 32, 1>, V<32, 8, 16, 1>, V<16, 16, 16, 1, 1>, V<16, 16, 8, 1, 1>>))
-SFD_VARIANTS(PF32, 512, (VL<V<32, 16, 1, 8>, V<32, 16, 1, 4>, V<16, 16, 1, 8, 0, 1, 2>, V<16, 16, 1, 4, 0, 1, 2>>),
-             (VL<V<32, 16, 16, 1>, V<32, 16, 8, 1>, V<16, 16, 16, 1, 0, 1, 2>, V<16, 16, 8, 1, 0, 1, 2>>))
+SFD_VARIANTS(PF32, 512,
+             (VL<V<32, 16, 1, 8>, V<32, 16, 1, 4>, V<16, 16, 1, 8, 0, 1, 2>, V<16, 16, 1, 4, 0, 1, 2>,
+                 V<32, 16, 1, 8, 1>, V<32, 16, 1, 4, 1>, V<32, 16, 1, 8, 0, 1, 1, 1>>),
+             (VL<V<32, 16, 16, 1>, V<32, 16, 8, 1>, V<16, 16, 16, 1, 0, 1, 2>, V<16, 16, 8, 1, 0, 1, 2>,
+                 V<32, 16, 16, 1, 1>, V<32, 16, 32, 1, 1>>))
 SFD_VARIANTS(PF32, 1024, (VL<V<32, 32, 1, 4>, V<32, 32, 1, 2>>), (VL<V<32, 32, 8, 1>, V<32, 32, 4, 1>>))
 SFD_VARIANTS(PF32, 2048, (VL<V<32, 32, 1, 2, 0, 1, 2>>), (VL<V<32, 32, 4, 1, 0, 1, 2>>))
 
@@ -147,6 +150,15 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
             }
             if constexpr (rt_twice(MODE) && V0::TS == 1) {
                 return false;  // the double round trip runs on a strided (y/z) axis only
+            } else if constexpr (rt_twice(MODE) && !rt_energy(MODE)) {
+                if (!a.V)
+                    hipLaunchKernelGGL(
+                        (k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W, V0::H, V0::T, true>), grid,
+                        block, 0, s, a);
+                else
+                    hipLaunchKernelGGL((k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W, V0::H, V0::T>),
+                                       grid, block, 0, s, a);
+                return true;
             } else {
                 hipLaunchKernelGGL((k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W, V0::H, V0::T>), grid,
                                    block, 0, s, a);
