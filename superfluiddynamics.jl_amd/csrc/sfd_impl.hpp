@@ -16,11 +16,16 @@ namespace sfd {
 //   [2m, 3m):  h = 2 only, the radix-2 stage's twC[k] = w_n^k, k < m
 // FP64 entries are (cos, sin) of the angle a; FP32 entries are the lifting
 // coefficients (tan(b/2), sin b) of b = a/2 with a taken in (-pi, pi].
+// With `centered` (FP32, h = 1, n1, n2 >= 2: the centered round trips of
+// the sweep kernels), [2m, 4m) repeats twA and twB for the centered indices
+// t_c = t - n2/2, q_c = q - n2/2, s_c = s - n1 [s >= n1/2], as the lifting
+// coefficients (tan(a/2), sin a) of the angle a itself, |a| <= pi/2.
 template <typename R>
-void fill_twiddles(std::vector<cx<R>>& w, int n, int n1, int n2, int h)
+void fill_twiddles(std::vector<cx<R>>& w, int n, int n1, int n2, int h, bool centered)
 {
     const int m = n1 * n2;
-    w.resize((size_t)(h == 2 ? 3 : 2) * m);
+    const bool ct = centered && sizeof(R) == 4 && h == 1 && n1 >= 2 && n2 >= 2;
+    w.resize((size_t)(ct ? 4 : (h == 2 ? 3 : 2)) * m);
     const long double pi = 3.14159265358979323846264338327950288L;
     // entry i = exp(-2 pi i e / d)
     auto put = [&](size_t i, long long e, long long d) {
@@ -42,6 +47,20 @@ void fill_twiddles(std::vector<cx<R>>& w, int n, int n1, int n2, int h)
         for (int s = 0; s < n1; ++s) put((size_t)m + (size_t)q * n1 + s, (long long)s * q, m);
     if (h == 2)
         for (int k = 0; k < m; ++k) put((size_t)2 * m + k, k, n);
+    if (ct) {
+        // exp(-2 pi i e / m) for a signed e with |e| <= m/4
+        auto put1 = [&](size_t i, long long e) {
+            const long double a = -2.0L * pi * (long double)e / (long double)m;
+            w[i].re = (R)tanl(0.5L * a);
+            w[i].im = (R)sinl(a);
+        };
+        auto sc = [&](int s) { return s >= n1 / 2 ? s - n1 : s; };
+        for (int s = 0; s < n1; ++s)
+            for (int t = 0; t < n2; ++t) put1((size_t)2 * m + (size_t)s * n2 + t, (long long)(t - n2 / 2) * sc(s));
+        for (int q = 0; q < n2; ++q)
+            for (int s = 0; s < n1; ++s)
+                put1((size_t)3 * m + (size_t)q * n1 + s, (long long)sc(s) * (q - n2 / 2));
+    }
 }
 
 // Bluestein tables of a line of n points on m >= 2n - 1 (appended to the
@@ -256,7 +275,7 @@ int set_variant(sfd_ctx* c, int axis, int idx)
         return SFD_OK;
     }
     std::vector<cx<R>> w;
-    fill_twiddles<R>(w, n, vi.n1, vi.n2, vi.h);
+    fill_twiddles<R>(w, n, vi.n1, vi.n2, vi.h, !c->blu[axis]);
     if (c->blu[axis]) fill_bluestein<R>(w, c->n[axis], n);
     if (c->tw[axis]) {
         SFD_HIP(hipStreamSynchronize(c->stream));
@@ -842,7 +861,7 @@ int batch_level_variant(sfd_batch* b, int L, int g, int ax, int idx)
         return fail(SFD_ERR_SHAPE, "batch: no sweep variant %d for line length %d", idx, G.n[ax]);
     if (!G.tw[L][ax] || G.vi[L][ax].n1 != vi.n1 || G.vi[L][ax].n2 != vi.n2 || G.vi[L][ax].h != vi.h) {
         std::vector<cx<R>> w;
-        fill_twiddles<R>(w, G.n[ax], vi.n1, vi.n2, vi.h);
+        fill_twiddles<R>(w, G.n[ax], vi.n1, vi.n2, vi.h, false);
         if (G.tw[L][ax]) {
             SFD_HIP(hipStreamSynchronize(b->stream));
             SFD_HIP(hipFree(G.tw[L][ax]));

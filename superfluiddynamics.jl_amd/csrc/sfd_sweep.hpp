@@ -315,17 +315,77 @@ __device__ __forceinline__ void store_line_all(const cx<R> (&v)[N1], cx<S>* psi,
     for (int j = 0; j < N1; ++j) lane_ref(b, j * g.jstride, g.lane) = narrow<S>(v[j]);
 }
 
+// Centered round trips (FP32 arithmetic, whole lines of N = N1*N2 points).
+// Lane t holds the positions N2 j + t_c (mod N) with t_c = t - N2/2, and the
+// stage-A frequencies s < N1 stand for s_c = s - N1 [s >= N1/2], so that
+// every inter-stage twiddle w_N^(t_c s_c) has |t_c s_c| <= N/4: an angle in
+// [-pi/2, pi/2], which one lifting rotation (tan(a/2) <= 1, 3 FMAs) applies
+// exactly invertibly, where the natural index ranges need two half-angle
+// lifts (6 FMAs).  Stage B then leaves (-1)^k2 X[s_c + N1 k2]: the sign is
+// undone by the inverse stage B of the same round trip, since the phase
+// multiply in between is diagonal in k.  Only element j = 0 of lanes
+// t < N2/2 wraps to the end of the line; ct_lane0 is its lane offset.
+template <typename R, int N1, int N2, int H>
+constexpr bool centered_ok = sizeof(R) == 4 && H == 1 && N1 >= 2 && N2 >= 2;
+
+template <int N, int N2>
+__device__ __forceinline__ unsigned ct_lane0(const TileGeom& g, int t)
+{
+    const long long shift = t < N2 / 2 ? N - N2 / 2 : -(N2 / 2);
+    return (unsigned)((long long)g.lane + shift * g.kstride);
+}
+
+template <typename S, typename R, int N1, int N2>
+__device__ __forceinline__ void load_line_ct(cx<R> (&v)[N1], const cx<S>* psi, const TileGeom& g, unsigned lane0)
+{
+    const cx<S>* b = psi + g.ubase;
+    v[0] = widen<R>(lane_ref(b, 0, lane0));
+#pragma unroll
+    for (int j = 1; j < N1; ++j) v[j] = widen<R>(lane_ref(b, j * g.jstride - (N2 / 2) * g.kstride, g.lane));
+}
+
+template <typename S, typename R, int N1, int N2>
+__device__ __forceinline__ void load_pot_ct(R (&vp)[N1], const S* V, const TileGeom& g, unsigned lane0)
+{
+    if (V) {
+        const S* b = V + g.ubase;
+        vp[0] = R(lane_ref(b, 0, lane0));
+#pragma unroll
+        for (int j = 1; j < N1; ++j) vp[j] = R(lane_ref(b, j * g.jstride - (N2 / 2) * g.kstride, g.lane));
+    } else {
+#pragma unroll
+        for (int j = 0; j < N1; ++j) vp[j] = R(0);
+    }
+}
+
+// ALL: also the lanes past the end of the field (bit-identical copies).
+template <typename S, typename R, int N1, int N2, bool ALL = false>
+__device__ __forceinline__ void store_line_ct(const cx<R> (&v)[N1], cx<S>* psi, const TileGeom& g, unsigned lane0)
+{
+    if (ALL || g.active) {
+        cx<S>* b = psi + g.ubase;
+        lane_ref(b, 0, lane0) = narrow<S>(v[0]);
+#pragma unroll
+        for (int j = 1; j < N1; ++j) lane_ref(b, j * g.jstride - (N2 / 2) * g.kstride, g.lane) = narrow<S>(v[j]);
+    }
+}
+
 // Forward transform of the tile's lines: phase A (N1-point DFTs, twiddles),
 // LDS exchange, phase B (N2-point DFTs).  On return w[r][k2] holds
 // X[s + N1 k2] of row s = t + N2 r.
-template <typename R, int N1, int N2, int TS, int G>
+// CT: centered index sets (see centered_ok): twA holds single-lift
+// coefficients of the centered angles.
+template <typename R, int N1, int N2, int TS, int G, bool CT = false>
 __device__ __forceinline__ void forward_ab(cx<R> (&v)[N1], cx<R> (&w)[N1 / N2][N2], cx<R>* lds, const cx<R>* twA,
                                            int c, int g, int t)
 {
     using Sh = SweepShape<R, N1, N2, TS, G>;
     dft<N1, -1, R>(v);
 #pragma unroll
-    for (int s = 1; s < N1; ++s) v[s] = tw_mul(v[s], twA[s * N2]);
+    for (int s = 1; s < N1; ++s) {
+        if constexpr (CT) v[s] = lift(v[s], twA[s * N2].re, twA[s * N2].im);
+        else v[s] = tw_mul(v[s], twA[s * N2]);
+    }
 #pragma unroll
     for (int s = 0; s < N1; ++s) lds[Sh::slot(c, g, s, t)] = v[s];
     lds_barrier();
@@ -345,7 +405,9 @@ __device__ __forceinline__ void forward_ab(cx<R> (&v)[N1], cx<R> (&w)[N1 / N2][N
 // may follow each other (in one tile or over consecutive tiles) with no
 // barrier in between: every cross-lane read sits behind the barrier that
 // follows the writes it needs.
-template <typename R, int N1, int N2, int TS, int G>
+// CT: centered index sets; column q = 0 stands for q_c = -N2/2 and takes a
+// twiddle too.
+template <typename R, int N1, int N2, int TS, int G, bool CT = false>
 __device__ __forceinline__ void inverse_ba(cx<R> (&w)[N1 / N2][N2], cx<R> (&v)[N1], cx<R>* lds, const cx<R>* twB,
                                            int c, int g, int t)
 {
@@ -356,7 +418,13 @@ __device__ __forceinline__ void inverse_ba(cx<R> (&w)[N1 / N2][N2], cx<R> (&v)[N
         const int s = t + N2 * r;
 #pragma unroll
         for (int q = 0; q < N2; ++q) {
-            const cx<R> z = (q == 0) ? w[r][q] : tw_mulc(w[r][q], twB[q * N1 + N2 * r]);
+            cx<R> z;
+            if constexpr (CT) {
+                const cx<R> b = twB[q * N1 + N2 * r];
+                z = lift(w[r][q], -b.re, -b.im);
+            } else {
+                z = (q == 0) ? w[r][q] : tw_mulc(w[r][q], twB[q * N1 + N2 * r]);
+            }
             lds[Sh::slot(c, g, s, q)] = z;
         }
     }
@@ -423,13 +491,18 @@ __device__ __forceinline__ void radix2_inv(cx<R> (&w)[N1 / N2][N2], const cx<R>*
 // H = 2: the line is two interleaved halves joined by the radix-2 stage; g is
 // then the thread's half-line group in LDS, t its sub-transform lane and h
 // its half; twC the radix-2 twiddles (unused for H = 1).
-template <typename S, typename R, int N1, int N2, int TS, int G, bool IMAG, bool EN = false, int H = 1>
+// CT: centered index sets (see centered_ok; twA, twB then point at the
+// centered tables, and lane0 is the wrapped lane offset of element 0).
+template <typename S, typename R, int N1, int N2, int TS, int G, bool IMAG, bool EN = false, int H = 1,
+          bool CT = false>
 __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1], const TileGeom& gm,
                                                 const SweepArgs& a, cx<R>* lds, const cx<R>* twA,
                                                 const cx<R>* twB, const cx<R>* twC, const cx<R>* phase, int c,
-                                                int g, int t, int h, double& norm_acc, double* eacc = nullptr)
+                                                int g, int t, int h, double& norm_acc, double* eacc = nullptr,
+                                                unsigned lane0 = 0)
 {
     static_assert(H == 1 || H == 2, "one or two halves per line");
+    static_assert(!CT || centered_ok<R, N1, N2, H>, "centered round trips: FP32 whole lines");
     constexpr int RB = N1 / N2;
     constexpr int M = N1 * N2;  // sub-transform length; the line has H * M points
     const int flags = a.flags;
@@ -451,8 +524,13 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
         }
     }
     cx<R> w[RB][N2];
-    forward_ab<R, N1, N2, TS, G>(v, w, lds, twA, c, g, t);
+    forward_ab<R, N1, N2, TS, G, CT>(v, w, lds, twA, c, g, t);
     if constexpr (H == 2) radix2_fwd<R, N1, N2, TS>(w, twC, h);
+    // spectrum index of w[r][k2]: s + N1 k2 with s = t + N2 r; centered, it
+    // is s_c + N1 k2 (mod M) = s + N1 ((k2 - d) mod N2), d = [s >= N1/2]:
+    // a compile-time shift for RB >= 2 (d = [r >= RB/2]), a lane-dependent
+    // one for RB = 1 (d = [t >= N2/2])
+    const int dct = (CT && RB == 1 && t >= N2 / 2) ? 1 : 0;
     if constexpr (EN) {
         // xi_k = kk * xi_1 with kk the signed frequency of k (the wavenumber
         // table's own product)
@@ -462,13 +540,28 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
         for (int r = 0; r < RB; ++r)
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2) {
-                const int k = t + M * h + N2 * r + N1 * k2;
+                const int d = RB == 1 ? dct : (r >= RB / 2 ? 1 : 0);
+                const int k = CT ? t + N2 * r + N1 * ((k2 - d) & (N2 - 1)) : t + M * h + N2 * r + N1 * k2;
                 const double x = (double)(k < N / 2 ? k : k - N) * dk;
                 const double re = w[r][k2].re, im = w[r][k2].im;
                 eacc[0] += x * x * (re * re + im * im);
             }
     }
+    // the phase entry of w[r][k2] at that index; for RB = 1 the k2 = 0 entry
+    // of lanes with d = 1 wraps to the end of the row
     const cx<R>* __restrict__ ph = phase + t + M * h;
+    const cx<R>* __restrict__ ph1 = ph - N1 * dct;                 // k2 >= 1
+    const cx<R>* __restrict__ ph0 = ph + dct * N1 * (N2 - 1);      // k2 = 0
+    auto phase_at = [&](int r, int k2) -> cx<R> {
+        if constexpr (!CT) {
+            return ph[N2 * r + N1 * k2];
+        } else if constexpr (RB == 1) {
+            return k2 == 0 ? ph0[0] : ph1[N1 * k2];
+        } else {
+            const int d = r >= RB / 2 ? 1 : 0;
+            return ph[N2 * r + N1 * ((k2 - d + N2) % N2)];
+        }
+    };
     if constexpr (sizeof(R) == 4) {
         // FP32: X m/N with X/N exact (N a power of two).  Real-time tables
         // hold the lifting coefficients (tan(b/2), sin b) of half the phase,
@@ -481,7 +574,7 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
         for (int r = 0; r < RB; ++r)
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2) {
-                const cx<R> d = ph[N2 * r + N1 * k2];
+                const cx<R> d = phase_at(r, k2);
                 const cx<R> x = w[r][k2];
                 if constexpr (IMAG)
                     w[r][k2] = {fmaf(x.re, d.re, x.re * inv_n), fmaf(x.im, d.re, x.im * inv_n)};
@@ -495,9 +588,12 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
             for (int k2 = 0; k2 < N2; ++k2) w[r][k2] = cmul(w[r][k2], ph[N2 * r + N1 * k2]);
     }
     if constexpr (H == 2) radix2_inv<R, N1, N2, TS>(w, twC, h);
-    inverse_ba<R, N1, N2, TS, G>(w, v, lds, twB, c, g, t);
+    inverse_ba<R, N1, N2, TS, G, CT>(w, v, lds, twB, c, g, t);
     if (flags & kFlagNL) {
-        if (a.save) store_line<S, R, N1>(v, static_cast<cx<S>*>(a.save), gm);
+        if (a.save) {
+            if constexpr (CT) store_line_ct<S, R, N1, N2>(v, static_cast<cx<S>*>(a.save), gm, lane0);
+            else store_line<S, R, N1>(v, static_cast<cx<S>*>(a.save), gm);
+        }
         const R beta = R(a.beta), dt = R(a.dt);
         const bool imag = sizeof(R) == 4 ? IMAG : (flags & kFlagImag) != 0;
 #pragma unroll
@@ -520,12 +616,13 @@ constexpr int kTwiddles = (H == 2 ? 3 : 2) * M;
 
 // Stages the twiddles (kTwiddles<M, H> entries) and one phase row (H*M
 // entries) into LDS, then waits for every lane.
-template <typename R, int M, int H, int NT>
+// CT: the centered twiddles, which follow the natural ones in the table.
+template <typename R, int M, int H, int NT, bool CT = false>
 __device__ __forceinline__ void stage_tables(cx<R>* tabs, const SweepArgs& a, int tid)
 {
     constexpr int NTW = kTwiddles<M, H>;
     constexpr int TOT = NTW + H * M;
-    const cx<R>* gtw = static_cast<const cx<R>*>(a.tw);
+    const cx<R>* gtw = static_cast<const cx<R>*>(a.tw) + (CT ? NTW : 0);
     const cx<R>* gph = static_cast<const cx<R>*>(a.phase);
 #pragma unroll
     for (int k = 0; k < (TOT + NT - 1) / NT; ++k) {
@@ -580,9 +677,14 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
 
     if constexpr (is_round_trip(MODE)) {
         constexpr bool EN = rt_energy(MODE);
-        load_line<S, R, N1>(v, psi, gm);
-        if ((flags & kFlagNL) || (EN && TS == 1 && (flags & kFlagPointwise)))
-            load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
+        constexpr bool CT = centered_ok<R, N1, N2, H>;
+        const unsigned lane0 = CT ? ct_lane0<N, N2>(gm, t) : 0u;
+        if constexpr (CT) load_line_ct<S, R, N1, N2>(v, psi, gm, lane0);
+        else load_line<S, R, N1>(v, psi, gm);
+        if ((flags & kFlagNL) || (EN && TS == 1 && (flags & kFlagPointwise))) {
+            if constexpr (CT) load_pot_ct<S, R, N1, N2>(vp, static_cast<const S*>(a.V), gm, lane0);
+            else load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
+        }
         double nacc = 0.0;
         double eacc[4] = {0, 0, 0, 0};
         if constexpr (EN || TL) {
@@ -594,18 +696,20 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
             // 0) they read the phase from global memory.
             constexpr int NTW = kTwiddles<M, H>;
             __shared__ cx<R> tabs[NTW + N];
-            stage_tables<R, M, H, Sh::NT>(tabs, a, tid);
+            stage_tables<R, M, H, Sh::NT, CT>(tabs, a, tid);
             if (EN || a.phase_row_stride == 0)
-                round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H>(
+                round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H, CT>(
                     v, vp, gm, a, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc,
-                    eacc);
+                    eacc, lane0);
             else
-                round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H>(
+                round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H, CT>(
                     v, vp, gm, a, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, row_phase<R>(a, gm), c, gh, t, h,
-                    nacc, eacc);
+                    nacc, eacc, lane0);
         } else {
-            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H>(
-                v, vp, gm, a, lds, twA, twB, twC, row_phase<R>(a, gm), c, gh, t, h, nacc);
+            constexpr int CTO = CT ? kTwiddles<M, H> : 0;  // the centered twiddles follow the natural ones
+            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H, CT>(
+                v, vp, gm, a, lds, twA + CTO, twB + CTO, twC, row_phase<R>(a, gm), c, gh, t, h, nacc, nullptr,
+                lane0);
         }
         if (flags & kFlagNorm) {
             double acc1[1] = {nacc};
@@ -618,7 +722,8 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
             }
             block_partials<R, Sh::NT, 4>(eacc, a.epartial, nblocks, tile);
         }
-        store_line<S, R, N1>(v, psi, gm);
+        if constexpr (CT) store_line_ct<S, R, N1, N2>(v, psi, gm, lane0);
+        else store_line<S, R, N1>(v, psi, gm);
     } else if constexpr (rt_twice(MODE)) {
         // K_d(tau) N(dt) K_d(tau) on the lines of one axis: the innermost
         // axis of a Strang step, whose flow commutes with the other axes'
@@ -629,12 +734,16 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
         // kRoundTripTwiceEnergy: the first round trip also sums the energy
         // of the loaded state (spectral sum only: this is never the x axis).
         constexpr bool EN = rt_energy(MODE);
-        load_line<S, R, N1>(v, psi, gm);
+        constexpr bool CT = centered_ok<R, N1, N2, H>;
+        const unsigned lane0 = CT ? ct_lane0<N, N2>(gm, t) : 0u;
+        if constexpr (CT) load_line_ct<S, R, N1, N2>(v, psi, gm, lane0);
+        else load_line<S, R, N1>(v, psi, gm);
         if constexpr (ZV) {
 #pragma unroll
             for (int j = 0; j < N1; ++j) vp[j] = R(0);
         } else if (flags & kFlagNL) {
-            load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
+            if constexpr (CT) load_pot_ct<S, R, N1, N2>(vp, static_cast<const S*>(a.V), gm, lane0);
+            else load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
         }
         double nacc = 0.0;
         double eacc[4] = {0, 0, 0, 0};
@@ -651,11 +760,13 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
         if constexpr (sizeof(R) == 4 || EN || TL) {
             constexpr int NTW = kTwiddles<M, H>;
             __shared__ cx<R> tabs[NTW + N];
-            stage_tables<R, M, H, Sh::NT>(tabs, a, tid);
-            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H>(
-                v, vp, gm, a1, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc, eacc);
-            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H>(
-                v, vp, gm, a2, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc);
+            stage_tables<R, M, H, Sh::NT, CT>(tabs, a, tid);
+            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H, CT>(
+                v, vp, gm, a1, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc, eacc,
+                lane0);
+            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H, CT>(
+                v, vp, gm, a2, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc,
+                nullptr, lane0);
         } else {
             const cx<R>* ph = row_phase<R>(a, gm);
             round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H>(v, vp, gm, a1, lds, twA, twB, twC,
@@ -671,7 +782,8 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
             if (!gm.active) eacc[0] = 0.0;
             block_partials<R, Sh::NT, 4>(eacc, a.epartial, nblocks, tile);
         }
-        store_line<S, R, N1>(v, psi, gm);
+        if constexpr (CT) store_line_ct<S, R, N1, N2>(v, psi, gm, lane0);
+        else store_line<S, R, N1>(v, psi, gm);
     } else if constexpr (MODE == kForward) {
         cx<R> w[RB][N2];
         load_line<S, R, N1>(v, psi, gm);
@@ -835,14 +947,19 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
     double nacc = 0.0;
     cx<R> v[N1];
     const R vp[N1] = {};
+    constexpr bool CT = centered_ok<R, N1, N2, 1>;
     TileGeom gm = tile_geom<N, N2, TS, NL>(a, tile, L, t);
-    if (tile < a.ntiles) load_line<S, R, N1>(v, psi, gm);
+    unsigned lane0 = CT ? ct_lane0<N, N2>(gm, t) : 0u;
+    if (tile < a.ntiles) {
+        if constexpr (CT) load_line_ct<S, R, N1, N2>(v, psi, gm, lane0);
+        else load_line<S, R, N1>(v, psi, gm);
+    }
     // stage the tables after the first tile's loads, so the loop starts with
     // no tile load outstanding
     {
         // unrolled, branch-free when NT divides 3N: the compiler then knows
         // every load above has completed once the tables are written
-        const cx<R>* gtw = static_cast<const cx<R>*>(a.tw);
+        const cx<R>* gtw = static_cast<const cx<R>*>(a.tw) + (CT ? 2 * N : 0);
         const cx<R>* gph = static_cast<const cx<R>*>(a.phase);
 #pragma unroll
         for (int k = 0; k < (3 * N + Sh::NT - 1) / Sh::NT; ++k) {
@@ -858,13 +975,21 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
         const long long next = tile + gridDim.x;
         const bool more = next < a.ntiles;
         TileGeom gn;
+        unsigned lane0n = 0u;
         cx<R> vn[N1];
         if (more) {
             gn = tile_geom<N, N2, TS, NL>(a, next, L, t);
-            load_line<S, R, N1>(vn, psi, gn);
+            if constexpr (CT) {
+                lane0n = ct_lane0<N, N2>(gn, t);
+                load_line_ct<S, R, N1, N2>(vn, psi, gn, lane0n);
+            } else {
+                load_line<S, R, N1>(vn, psi, gn);
+            }
         }
-        round_trip_tile<S, R, N1, N2, TS, G, IMAG>(v, vp, gm, a, lds, twA, twB, nullptr, ph, c, g, t, 0, nacc);
-        store_line_all<S, R, N1>(v, psi, gm);
+        round_trip_tile<S, R, N1, N2, TS, G, IMAG, false, 1, CT>(v, vp, gm, a, lds, twA, twB, nullptr, ph, c, g, t,
+                                                                   0, nacc, nullptr, lane0);
+        if constexpr (CT) store_line_ct<S, R, N1, N2, true>(v, psi, gm, lane0);
+        else store_line_all<S, R, N1>(v, psi, gm);
         if (!more) break;
         // the next tile's loads were issued before these N1 stores: wait for
         // them here, with the stores still in flight
@@ -872,6 +997,7 @@ __global__ __launch_bounds__(TS * G * N2) void k_sweep_pipe(const SweepArgs a)
 #pragma unroll
         for (int j = 0; j < N1; ++j) v[j] = vn[j];
         gm = gn;
+        lane0 = lane0n;
         tile = next;
     }
     if (flags & kFlagNorm) {
