@@ -44,12 +44,12 @@ def bytes_per_site_update(prec: str, ndim: int = 3, with_v: bool = False, passes
     return passes * 2 * sc + (sr if with_v else 0)
 
 
-def load_traffic(slot: str):
-    """HBM bytes per launch of the kernel of sweep slot `slot` from the
-    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy-kernel calibration recorded with it); None if it holds no figure
-    for that kernel."""
-    path = os.path.join(ROOT, "profiles", "traffic_256cube_f32.json")
+def load_traffic(slot: str, n: int = 256):
+    """HBM bytes per launch of the kernel of sweep slot `slot` on the n³ FP32
+    workload from the committed rocprofv3 --pmc summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration recorded with it); None
+    if there is no summary for n or it holds no figure for that kernel."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{n}cube_f32.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -385,7 +385,7 @@ def main(argv=None):
     dom_ms = res["f32"]["back_to_back"].get(dom, k32[dom]["avg_ms"])
     launch_bytes = 2 * 8 * sites
     achieved = launch_bytes / (dom_ms * 1e-3)
-    traffic = load_traffic(dom)
+    traffic = load_traffic(dom, n)
     dom_kernel = kernel_name(res["f32"]["variants"], SWEEP_SLOTS.index(dom))
 
     # Accuracy at the largest stable step: at dt = 5e-4 the unfiltered split is

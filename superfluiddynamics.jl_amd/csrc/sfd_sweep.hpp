@@ -687,13 +687,20 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
         }
         double nacc = 0.0;
         double eacc[4] = {0, 0, 0, 0};
-        if constexpr (EN || TL) {
+        if constexpr (EN || TL || (CT && (RB == 1 || TS == 1))) {
             // Energy sweeps read the twiddle and phase tables (one phase row:
             // Omega = 0) from LDS: with the energy sums' registers live the
             // compiler no longer batches the tables' global loads and would
             // wait on each in turn.  TL variants do so for the same reason
-            // (FP64 lines of 512 points); with several phase rows (Omega !=
-            // 0) they read the phase from global memory.
+            // (FP64 lines of 512 points), and so do the centered FP32 round
+            // trips with N1 = N2, whose lane-dependent phase index defeats
+            // that batching too (a strided 256-point round trip at 256^3: 46
+            // us from LDS, 85 us from global memory), and those on the
+            // contiguous axis (512 points, 4 lines per workgroup: 453 against
+            // 959 us at 512^3).  Strided centered round trips with N1 > N2
+            // keep them in global memory (512 points: 513 against 666 us).
+            // With several phase rows (Omega != 0) the phase is read from
+            // global memory.
             constexpr int NTW = kTwiddles<M, H>;
             __shared__ cx<R> tabs[NTW + N];
             stage_tables<R, M, H, Sh::NT, CT>(tabs, a, tid);

@@ -21,13 +21,14 @@ def test_bytes_per_site_update(prec, ndim, with_v, passes, want):
     assert bench.bytes_per_site_update(prec, ndim, with_v, passes) == want
 
 
+@pytest.mark.parametrize("n", [256, 512])
 @pytest.mark.parametrize("slot", ["x round trip", "y round trip", "z round trip", "double round trip [d N d]"])
-def test_committed_traffic_is_close_to_the_algorithmic_bytes(slot):
+def test_committed_traffic_is_close_to_the_algorithmic_bytes(slot, n):
     """The calibrated counter bytes of every kernel of the step sit within 1 %
-    above one read and one write of the 256³ complex FP32 field (no re-reads)."""
-    t = bench.load_traffic(slot)
+    above one read and one write of the n³ complex FP32 field (no re-reads)."""
+    t = bench.load_traffic(slot, n)
     assert t is not None
-    alg = 2 * 8 * 256 ** 3
+    alg = 2 * 8 * n ** 3
     assert alg <= t < 1.01 * alg
 
 
@@ -37,6 +38,7 @@ def test_traffic_summary_is_plain_json():
         d = json.load(fh)
     assert d["algorithmic_bytes_per_launch"] == 2 * 8 * 256 ** 3
     assert bench.load_traffic("other sweeps") is None
+    assert bench.load_traffic("x round trip", 128) is None
 
 
 def test_kernel_names_of_the_sweep_slots():
