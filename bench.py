@@ -140,10 +140,26 @@ def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, reps, launches, settle_ms
             torch.cuda.synchronize()
             reps.append(k0.elapsed_time(k1) / launches)
         back_to_back[name] = sorted(reps)[1]
-    del model, f
+    # A plain elementwise pass over the same field, timed the same way: torch's
+    # in-place mul_ (one read and one write of every byte), the rate a kernel
+    # that only streams the field reaches on this GPU at this size.
+    flat = torch.view_as_real(f.ϕ)
+    reps = []
+    for _ in range(3):
+        k0 = torch.cuda.Event(enable_timing=True)
+        k1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        k0.record()
+        for _ in range(launches):
+            flat.mul_(1.0)
+        k1.record()
+        torch.cuda.synchronize()
+        reps.append(k0.elapsed_time(k1) / launches)
+    copy_ms = sorted(reps)[1]
+    del model, f, flat
     torch.cuda.empty_cache()
     return dict(ms=ms_step, runs_ms=runs, state=state, norm=nrm, variants=variants, tune=tune, solve=solve,
-                sweeps=sweeps, back_to_back=back_to_back, settle_steps=settle_steps)
+                sweeps=sweeps, back_to_back=back_to_back, copy_ms=copy_ms, settle_steps=settle_steps)
 
 
 def solve_loop(sfd, torch, model, k):
@@ -387,6 +403,7 @@ def main(argv=None):
     achieved = launch_bytes / (dom_ms * 1e-3)
     traffic = load_traffic(dom, n)
     dom_kernel = kernel_name(res["f32"]["variants"], SWEEP_SLOTS.index(dom))
+    copy_ms = res["f32"]["copy_ms"]
 
     # Accuracy at the largest stable step: at dt = 5e-4 the unfiltered split is
     # past its stability limit (D |coeffD| kmax^2 dt = 12.3 > pi) and any two
@@ -468,7 +485,12 @@ def main(argv=None):
                      "bytes_per_launch": launch_bytes,
                      "timing": "avg_launch_ms: HIP events around back-to-back launches on the context's stream; "
                                "avg_launch_ms_in_step: events around each launch inside the timed steps "
-                               "(adds a few us per launch)"},
+                               "(adds a few us per launch)",
+                     "copy_reference": {
+                         "kernel": "torch in-place mul_ over the same complex FP32 field (one read + one write)",
+                         "avg_launch_ms": copy_ms, "GB/s": launch_bytes / (copy_ms * 1e-3) / 1e9,
+                         "frac": launch_bytes / (copy_ms * 1e-3) / HBM_PEAK,
+                         "dominant_over_copy": copy_ms / dom_ms}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
