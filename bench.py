@@ -44,6 +44,16 @@ def bytes_per_site_update(prec: str, ndim: int = 3, with_v: bool = False, passes
     return passes * 2 * sc + (sr if with_v else 0)
 
 
+def passes_per_step(ndim: int, steps: int, chained: bool = True) -> float:
+    """Sweeps per Strang step of one call of `steps` steps (Omega = 0, real
+    time): x, y, [z N z], x, y per step (2D - 1) when each step runs alone;
+    x, y, [z N z], y, [x x], y, [z N z], ..., y, x when the call chains them
+    ((2D - 2) per step plus the closing x sweep)."""
+    if not chained or steps < 2:
+        return float(2 * ndim - 1)
+    return ((2 * ndim - 2) * steps + 1) / steps
+
+
 def load_traffic(slot: str, n: int = 256):
     """HBM bytes per launch of the kernel of sweep slot `slot` on the n³ FP32
     workload from the committed rocprofv3 --pmc summary (FETCH_SIZE +
@@ -61,7 +71,8 @@ def load_traffic(slot: str, n: int = 256):
 PRECISIONS = {"f32": ("float32", None), "f32x": ("float32", "float64"), "f64": ("float64", None)}
 
 
-SWEEP_SLOTS = ("x round trip", "y round trip", "z round trip", "double round trip [d N d]", "other sweeps")
+SWEEP_SLOTS = ("x round trip", "y round trip", "z round trip", "double round trip [d N d]", "other sweeps",
+               "double round trip [x x]")
 
 
 def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, reps, launches, settle_ms=200.0, autotune=True):
@@ -111,8 +122,8 @@ def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, reps, launches, settle_ms
     sfd._lib.check(lib.sfd_ctx_time_sweeps(ctx.h, 1))
     sfd._lib.check(lib.sfd_ctx_sweep_times(ctx.h, None, None, 1))
     model.steps(steps)
-    tot = (ctypes.c_double * 5)()
-    cnt = (ctypes.c_int64 * 5)()
+    tot = (ctypes.c_double * len(SWEEP_SLOTS))()
+    cnt = (ctypes.c_int64 * len(SWEEP_SLOTS))()
     sfd._lib.check(lib.sfd_ctx_sweep_times(ctx.h, tot, cnt, 1))
     sfd._lib.check(lib.sfd_ctx_time_sweeps(ctx.h, 0))
     sweeps = {}
@@ -182,11 +193,12 @@ def solve_loop(sfd, torch, model, k):
 
 def kernel_name(variants, slot):
     """Kernel of sweep slot `slot` (0-2: the round trip of axis x, y, z; 3: the
-    double round trip of the innermost axis) under the FP32 variants
-    (n1, n2, ts, g, pipelined, waves, halves) per axis."""
-    ax = 2 if slot == 3 else slot
+    double round trip of the innermost axis; 5: the double round trip of x)
+    under the FP32 variants (n1, n2, ts, g, pipelined, waves, halves) per
+    axis."""
+    ax = {3: 2, 5: 0}.get(slot, slot)
     n1, n2, ts, g, pipe = variants[ax][:5]
-    if slot == 3:
+    if slot in (3, 5):
         return f"k_sweep<float, float, {n1}, {n2}, {ts}, {g}, 7>"
     return f"{'k_sweep_pipe' if pipe else 'k_sweep'}<float, float, {n1}, {n2}, {ts}, {g}>"
 
@@ -384,7 +396,7 @@ def main(argv=None):
         r = res[prec]
         sups = sites / (r["ms"] * 1e-3)
         p_ = "f64" if prec == "f64" else "f32"
-        b = bytes_per_site_update(p_, passes=5)
+        b = bytes_per_site_update(p_, passes=passes_per_step(3, args.steps))
         b6 = bytes_per_site_update(p_)
         out[prec] = dict(site_updates_per_s=sups, ms_per_step=r["ms"], runs_ms=r["runs_ms"],
                          bytes_per_site_update=b, step_hbm_frac=sups * b / HBM_PEAK,
@@ -454,8 +466,10 @@ def main(argv=None):
         "f32_over_f64_speedup": out["f64"]["ms_per_step"] / out["f32"]["ms_per_step"],
         "f32x_over_f64_speedup": out["f64"]["ms_per_step"] / out["f32x"]["ms_per_step"],
         "f32_step_hbm_frac": out["f32"]["step_hbm_frac"],
-        "f32_step_hbm_frac_note": "the step moves 5 passes x 16 B = 80 B per site-update (x, y, [z N z], x, y); "
-                                  "priced at the 6-pass 96 B of the reference's structure it would read "
+        "f32_step_hbm_frac_note": f"a call of {args.steps} chained steps (x, y, [z N z], y, [x x], y, ..., y, x) "
+                                  f"moves {passes_per_step(3, args.steps):.3f} passes x 16 B = "
+                                  f"{out['f32']['bytes_per_site_update']:.2f} B per site-update; priced at the "
+                                  "6-pass 96 B of the reference's structure it would read "
                                   f"{out['f32']['step_hbm_frac_6_pass_bytes']:.3f}",
         "f32_kernels": out["f32"]["kernels"],
         "f32_back_to_back_ms": res["f32"]["back_to_back"],

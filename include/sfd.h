@@ -69,7 +69,15 @@ int sfd_ctx_set_stream(sfd_ctx* ctx, void* stream);
  * and solveNL! (splitting.jl:205-209), for nsteps consecutive calls.
  * V may be NULL (PotentialZero).  mode SFD_IMAG_TIME runs the imaginary-time
  * variant: every exponent loses its factor i and the field is renormalised
- * to norm 1 (normalize!, Fields.jl:196-200) after each step. */
+ * to norm 1 (normalize!, Fields.jl:196-200) after each step.
+ * Sweep order of a Strang step: the innermost axis's two half-step flows
+ * commute with the other axes' and meet around the nonlinear phase in one
+ * pass, x, y, [z N z], x, y (2D without rotation: x, [y N y], x).  In real
+ * time with Omega = 0 every flow commutes, and a call of several steps runs
+ * each step as x, y, [z N z], y, x, so that the x flows of consecutive steps
+ * share one pass, [x x]: 2D - 2 passes over the field per step.  Each
+ * half-step flow keeps its own round trip; only the order of commuting flows
+ * changes, so the results equal the reference order's up to rounding. */
 int sfd_split_step(sfd_ctx* ctx, void* psi, const void* V, double coeffD, double beta,
                    double Omega, double dt, int64_t nsteps, int order, int mode);
 
@@ -163,10 +171,11 @@ int sfd_ctx_set_variant(sfd_ctx* ctx, int axis, int index);
  * sweep launch of the context is bracketed by two HIP events on its stream
  * and its duration summed per slot: 0, 1, 2 = one-axis round trips of x, y,
  * z; 3 = the double round trip [d N d] of a Strang step; 4 = forward,
- * inverse and energy sweeps.  The events cost a few microseconds between
- * launches, so time whole steps with the timer off. */
+ * inverse and energy sweeps; 5 = the double round trip [x x] of chained
+ * Strang steps.  The events cost a few microseconds between launches, so
+ * time whole steps with the timer off. */
 int sfd_ctx_time_sweeps(sfd_ctx* ctx, int enable);
-/* ms_sum[5] and count[5] (either may be NULL) receive the summed durations
+/* ms_sum[6] and count[6] (either may be NULL) receive the summed durations
  * and launch counts per slot since the last reset; reset != 0 clears them.
  * Synchronises on the recorded events. */
 int sfd_ctx_sweep_times(sfd_ctx* ctx, double* ms_sum, int64_t* count, int reset);

@@ -743,7 +743,7 @@ int sfd_split_step(sfd_ctx* ctx, void* psi, const void* V, double coeffD, double
     if (nsteps < 0) return fail(SFD_ERR_ARG, "nsteps must be >= 0");
     if (!std::isfinite(coeffD) || !std::isfinite(beta) || !std::isfinite(Omega) || !std::isfinite(dt))
         return fail(SFD_ERR_ARG, "non-finite scalar argument");
-    return ops_for(ctx->prec).split(ctx, psi, V, coeffD, beta, Omega, dt, nsteps, order, mode, nullptr, true, nullptr);
+    return ops_for(ctx->prec).split(ctx, psi, V, coeffD, beta, Omega, dt, nsteps, order, mode, nullptr, kFoldChain, nullptr);
 }
 
 int sfd_split_step_energy(sfd_ctx* ctx, void* psi, const void* V, double coeffD, double beta, double Omega,
@@ -1083,7 +1083,7 @@ int sfd_split_step_noalloc(sfd_ctx* ctx, void* psi, const void* V, double coeffD
         ws.phase[ax] = base + tab_off[ax];
         SFD_TRY(ops.build_phase_into(ctx, ctx->stream, ws.phase[ax], ax, tau, coeffD, Omega, imag, ws.rows[ax]));
     }
-    return ops.split(ctx, psi, V, coeffD, beta, Omega, dt, nsteps, order, mode, nullptr, true, &ws);
+    return ops.split(ctx, psi, V, coeffD, beta, Omega, dt, nsteps, order, mode, nullptr, kFoldChain, &ws);
 }
 
 int sfd_ctx_time_sweeps(sfd_ctx* ctx, int enable)

@@ -233,7 +233,8 @@ int launch_sweep(sfd_ctx* c, int axis, SweepArgs a)
     Launch L = sweep_geometry(c, axis, a, persistent && !blu);
     a.tw = c->tw[axis];
     a.nlen = c->n[axis];
-    if (c->time_sweeps) SFD_TRY(timer_begin(c, rt_twice(MODE) ? 3 : (is_round_trip(MODE) ? axis : 4)));
+    if (c->time_sweeps)
+        SFD_TRY(timer_begin(c, rt_twice(MODE) ? (axis == 0 ? 5 : 3) : (is_round_trip(MODE) ? axis : 4)));
     bool ok = false;
     if constexpr (rt_energy(MODE)) {
         if (blu) return fail(SFD_ERR_SHAPE, "no energy sums fused into Bluestein sweeps (n = %d)", c->n[axis]);
@@ -383,10 +384,19 @@ template <typename P>
 // so this holds for z in 3D, and for y in 2D only when Omega = 0), so the
 // second half step K_z K_y K_x may apply z first, and the two z flows meet
 // around the nonlinear phase in one double round-trip sweep (2D - 1 sweeps
-// per step instead of 2D).  `fold = false` keeps the x, y, z, N, x, y, z
+// per step instead of 2D).  `fold = kFoldNone` keeps the x, y, z, N, x, y, z
 // order of splitting.jl:149-154.
+//
+// With `fold = kFoldChain`, real time, Omega = 0 and several steps, every
+// flow commutes with every other, so each step may run its second half in
+// reverse, x, y, [z N z], y, x; the x flow ending step s and the one
+// starting step s + 1 then meet in one double round trip [x x] (two round
+// trips of tau each, in one read and one write of the field): x, y,
+// [z N z], y, [x x], y, [z N z], ..., y, x, i.e. 2D - 2 sweeps per step
+// instead of 2D - 1 (2D: x, [y N y], [x x], [y N y], ..., x).  Every
+// half-step flow is still applied by its own round trip.
 int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt,
-               int64_t nsteps, int order, int mode, double* ehist = nullptr, bool fold = true,
+               int64_t nsteps, int order, int mode, double* ehist = nullptr, int fold = kFoldInner,
                const Workspace* ws = nullptr)
 {
     using S = typename P::S;
@@ -413,7 +423,33 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
         for (int ax = 0; ax < D; ++ax) epart[ax] = c->epartial + off[ax];
     }
     bool pending = false;
-    fold = fold && order == 2 && D >= 2 && (Om == 0.0 || D == 3);
+    if (!(order == 2 && D >= 2 && (Om == 0.0 || D == 3))) fold = kFoldNone;
+    if (fold == kFoldChain) {
+        if (!imag && Om == 0.0 && !ehist && nsteps >= 2 && !c->blu[0]) {
+            // x, y (first half of step 0)
+            for (int ax = 0; ax < D - 1; ++ax)
+                SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt, nullptr, nullptr, false, ws));
+            for (int64_t s = 0; s < nsteps; ++s) {
+                SFD_TRY(round_trip<P>(c, psi, V, D - 1, tau, cD, Om, 0, nlflags, beta, dt, nullptr, nullptr, true,
+                                      ws));
+                // second half in reverse down to y
+                for (int ax = D - 2; ax >= 1; --ax)
+                    SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt, nullptr, nullptr, false, ws));
+                if (s + 1 < nsteps) {
+                    // x of step s and x of step s + 1: no nonlinear phase, no potential
+                    SFD_TRY(round_trip<P>(c, psi, nullptr, 0, tau, cD, Om, 0, 0, beta, dt, nullptr, nullptr, true,
+                                          ws));
+                    for (int ax = 1; ax < D - 1; ++ax)
+                        SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt, nullptr, nullptr, false,
+                                              ws));
+                } else {
+                    SFD_TRY(round_trip<P>(c, psi, V, 0, tau, cD, Om, 0, 0, beta, dt, nullptr, nullptr, false, ws));
+                }
+            }
+            return SFD_OK;
+        }
+        fold = kFoldInner;
+    }
     for (int64_t s = 0; s < nsteps && fold; ++s) {
         FinalizeSets fs{};
         auto record = [&](int ax) {
@@ -769,16 +805,17 @@ int autotune_impl(sfd_ctx* c, void* scratch, float* times_out)
         float best_ms = 1e30f;
         // The innermost axis of a Strang step runs the double round trip
         // [d N d] (z in 3D; y in 2D without rotation) besides its single
-        // round trips (Split1, rotating 2D steps, transforms): its variant is
+        // round trips (Split1, rotating 2D steps, transforms), and the x axis
+        // the double round trip [x x] of chained steps: their variants are
         // chosen by the sum of both times.
-        const bool folds = c->ndim >= 2 && ax == c->ndim - 1;
+        const bool folds = c->ndim >= 2 && (ax == c->ndim - 1 || (ax == 0 && !c->blu[0]));
         for (int v = 0; rc == SFD_OK && v < nv; ++v) {
             if ((rc = set_variant<P>(c, ax, v)) != SFD_OK) break;
             float score = 0.f;
             for (int twice = 0; rc == SFD_OK && twice <= (folds ? 1 : 0); ++twice) {
                 auto launch = [&]() {
-                    return twice ? round_trip<P>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, kFlagNL, 0.0, 0.0,
-                                                 nullptr, nullptr, true)
+                    return twice ? round_trip<P>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, ax == 0 ? 0 : kFlagNL, 0.0,
+                                                 0.0, nullptr, nullptr, true)
                                  : round_trip<P>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, 0, 0.0, 0.0);
                 };
                 float vbest = 1e30f;
@@ -1227,7 +1264,7 @@ int round_trip_axis(sfd_ctx* c, void* psi, int axis, double tau, double cD, doub
 
 template <typename P>
 int split_op(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt, int64_t nsteps,
-             int order, int mode, double* ehist, bool fold, const Workspace* ws)
+             int order, int mode, double* ehist, int fold, const Workspace* ws)
 {
     return split_impl<P>(c, psi, V, cD, beta, Om, dt, nsteps, order, mode, ehist, fold, ws);
 }

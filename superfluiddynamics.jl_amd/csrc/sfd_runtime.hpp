@@ -152,6 +152,15 @@ struct Workspace {
     int rows[3];
 };
 
+// Sweep orders of the split step (split_impl):
+//   kFoldNone   x, y, z, N, x, y, z: the order of splitting.jl:149-154;
+//   kFoldInner  x, y, [z N z], x, y: the innermost axis's two flows meet
+//               around the nonlinear phase in one double round trip;
+//   kFoldChain  also x, y, [z N z], y, x per step, so that the last x flow
+//               of a step and the first of the next meet in one double
+//               round trip [x x] (real time, Omega = 0, several steps).
+enum FoldLevel : int { kFoldNone = 0, kFoldInner = 1, kFoldChain = 2 };
+
 }  // namespace sfd
 
 struct sfd_ctx {
@@ -187,8 +196,9 @@ struct sfd_ctx {
     bool pin_phases = false;    // no eviction: two streams read the cache
     // Sweep timer (sfd_ctx_time_sweeps): a HIP event before and after every
     // sweep launch, summed per slot: 0..2 one-axis round trips of x, y, z,
-    // 3 the double round trip [d N d], 4 forward/inverse/energy sweeps.
-    static constexpr int kTimerSlots = 5;
+    // 3 the double round trip [d N d], 4 forward/inverse/energy sweeps,
+    // 5 the double round trip [x x] of chained Strang steps.
+    static constexpr int kTimerSlots = 6;
     static constexpr int kTimerPairs = 128;
     bool time_sweeps = false;
     std::vector<hipEvent_t> tev;  // 2 * kTimerPairs events, created on enable
@@ -370,7 +380,7 @@ void batch_release(sfd_batch* b);
 struct PrecOps {
     int (*init_tables)(sfd_ctx* c);
     int (*split)(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt, int64_t nsteps,
-                 int order, int mode, double* ehist, bool fold, const Workspace* ws);
+                 int order, int mode, double* ehist, int fold, const Workspace* ws);
     int (*split_energy)(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt,
                         int64_t nsteps, int order, int mode, double* energies);
     int (*round_trip_axis)(sfd_ctx* c, void* psi, int axis, double tau, double cD, double Om, int imag);

@@ -149,7 +149,17 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
                 }
             }
             if constexpr (rt_twice(MODE) && V0::TS == 1) {
-                return false;  // the double round trip runs on a strided (y/z) axis only
+                // the x axis runs only the double round trip [x x] of chained
+                // Strang steps: real time, no nonlinear phase, no potential
+                if constexpr (MODE == kRoundTripTwice) {
+                    if (a.V || (a.flags & (kFlagNL | kFlagScale | kFlagNorm))) return false;
+                    hipLaunchKernelGGL(
+                        (k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W, V0::H, V0::T, true>), grid,
+                        block, 0, s, a);
+                    return true;
+                } else {
+                    return false;
+                }
             } else if constexpr (rt_twice(MODE) && !rt_energy(MODE)) {
                 if (!a.V)
                     hipLaunchKernelGGL(

@@ -21,6 +21,17 @@ def test_bytes_per_site_update(prec, ndim, with_v, passes, want):
     assert bench.bytes_per_site_update(prec, ndim, with_v, passes) == want
 
 
+@pytest.mark.parametrize("ndim, steps, chained, want", [
+    (3, 1, True, 5.0),         # one step alone: x, y, [z N z], y, x
+    (3, 2, True, 4.5),         # x, y, [z N z], y, [x x], y, [z N z], y, x: 9 sweeps
+    (3, 200, True, 4.005),
+    (3, 200, False, 5.0),
+    (2, 4, True, 2.25),        # x, [y N y], [x x], [y N y], [x x], ..., x: 9 sweeps
+])
+def test_passes_per_step(ndim, steps, chained, want):
+    assert bench.passes_per_step(ndim, steps, chained) == pytest.approx(want)
+
+
 @pytest.mark.parametrize("n", [256, 512])
 @pytest.mark.parametrize("slot", ["x round trip", "y round trip", "z round trip", "double round trip [d N d]"])
 def test_committed_traffic_is_close_to_the_algorithmic_bytes(slot, n):
