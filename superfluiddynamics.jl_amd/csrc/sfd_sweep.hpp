@@ -493,8 +493,13 @@ __device__ __forceinline__ void radix2_inv(cx<R> (&w)[N1 / N2][N2], const cx<R>*
 // its half; twC the radix-2 twiddles (unused for H = 1).
 // CT: centered index sets (see centered_ok; twA, twB then point at the
 // centered tables, and lane0 is the wrapped lane offset of element 0).
+// SC (FP32 real time): the spectrum's factor 1/N^SC, SC = 0, 1 or 2; N is a
+// power of two, so the factor is exact and may be deferred: the double round
+// trip applies none in its first round trip and 1/N^2 in its second, with
+// beta/N^2 in the nonlinear phase between them (beta/N^2 |N psi|^2 rounds
+// to the same float as beta |psi|^2).
 template <typename S, typename R, int N1, int N2, int TS, int G, bool IMAG, bool EN = false, int H = 1,
-          bool CT = false>
+          bool CT = false, int SC = 1>
 __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1], const TileGeom& gm,
                                                 const SweepArgs& a, cx<R>* lds, const cx<R>* twA,
                                                 const cx<R>* twB, const cx<R>* twC, const cx<R>* phase, int c,
@@ -569,7 +574,8 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
         // by rounded (cos phi, sin phi) loses 1 - cos(phi) of the low modes
         // to rounding on every pass and drifts the norm.  Imaginary-time
         // tables hold d = (m - 1)/N (real): X/N + X d.
-        constexpr float inv_n = 1.0f / float(H * M);
+        static_assert(SC == 1 || !IMAG, "imaginary-time tables hold the factor 1/N");
+        constexpr float inv_n = SC == 2 ? 1.0f / (float(H * M) * float(H * M)) : 1.0f / float(H * M);
 #pragma unroll
         for (int r = 0; r < RB; ++r)
 #pragma unroll
@@ -578,6 +584,8 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
                 const cx<R> x = w[r][k2];
                 if constexpr (IMAG)
                     w[r][k2] = {fmaf(x.re, d.re, x.re * inv_n), fmaf(x.im, d.re, x.im * inv_n)};
+                else if constexpr (SC == 0)
+                    w[r][k2] = lift(lift(x, d.re, d.im), d.re, d.im);
                 else
                     w[r][k2] = lift(lift({x.re * inv_n, x.im * inv_n}, d.re, d.im), d.re, d.im);
             }
@@ -591,8 +599,11 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
     inverse_ba<R, N1, N2, TS, G, CT>(w, v, lds, twB, c, g, t);
     if (flags & kFlagNL) {
         if (a.save) {
-            if constexpr (CT) store_line_ct<S, R, N1, N2>(v, static_cast<cx<S>*>(a.save), gm, lane0);
-            else store_line<S, R, N1>(v, static_cast<cx<S>*>(a.save), gm);
+            cx<R> u[N1];
+#pragma unroll
+            for (int j = 0; j < N1; ++j) u[j] = SC == 0 ? cscale(v[j], R(1) / R(H * M)) : v[j];
+            if constexpr (CT) store_line_ct<S, R, N1, N2>(u, static_cast<cx<S>*>(a.save), gm, lane0);
+            else store_line<S, R, N1>(u, static_cast<cx<S>*>(a.save), gm);
         }
         const R beta = R(a.beta), dt = R(a.dt);
         const bool imag = sizeof(R) == 4 ? IMAG : (flags & kFlagImag) != 0;
@@ -768,10 +779,15 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
             constexpr int NTW = kTwiddles<M, H>;
             __shared__ cx<R> tabs[NTW + N];
             stage_tables<R, M, H, Sh::NT, CT>(tabs, a, tid);
-            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H, CT>(
+            // FP32 real time: the first round trip leaves N psi (no 1/N),
+            // the second applies 1/N^2 (see round_trip_tile's SC)
+            constexpr int SC1 = (sizeof(R) == 4 && !rt_imag(MODE)) ? 0 : 1;
+            constexpr int SC2 = SC1 == 0 ? 2 : 1;
+            if constexpr (SC1 == 0) a1.beta = a.beta / ((double)N * (double)N);
+            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H, CT, SC1>(
                 v, vp, gm, a1, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc, eacc,
                 lane0);
-            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H, CT>(
+            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H, CT, SC2>(
                 v, vp, gm, a2, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc,
                 nullptr, lane0);
         } else {

@@ -75,7 +75,8 @@ SFD_VARIANTS(PF32, 32, (VL<V<8, 4, 1, 64>>), (VL<V<8, 4, 16, 1>>))
 SFD_VARIANTS(PF32, 64, (VL<V<8, 8, 1, 32>, V<16, 4, 1, 64>>), (VL<V<8, 8, 16, 1>, V<8, 8, 32, 1>>))
 SFD_VARIANTS(PF32, 128, (VL<V<16, 8, 1, 32>, V<16, 8, 1, 16>, V<32, 4, 1, 32>, V<16, 8, 1, 16, 1>>),
              (VL<V<16, 8, 16, 1>, V<16, 8, 32, 1>, V<32, 4, 32, 1>, V<16, 8, 16, 1, 1>>))
-SFD_VARIANTS(PF32, 256, (VL<V<16, 16, 1, 16>, V<16, 16, 1, 8>, V<32, 8, 1, 16>, V<16, 16, 1, 16, 1>, V<16, 16, 1, 8, 1>>),
+SFD_VARIANTS(PF32, 256, (VL<V<16, 16, 1, 16>, V<16, 16, 1, 8>, V<32, 8, 1, 16>, V<16, 16, 1, 16, 1>, V<16, 16, 1, 8, 1>,
+                            V<16, 16, 1, 32>, V<16, 16, 1, 32, 1>>),
              (VL<V<16, 16, 16, 1>, V<16, 16, 32, 1>, V<32, 8, 16, 1>, V<16, 16, 16, 1, 1>, V<16, 16, 8, 1, 1>>))
 SFD_VARIANTS(PF32, 512,
              (VL<V<32, 16, 1, 8>, V<32, 16, 1, 4>, V<16, 16, 1, 8, 0, 1, 2>, V<16, 16, 1, 4, 0, 1, 2>,
