@@ -32,8 +32,9 @@ def test_passes_per_step(ndim, steps, chained, want):
     assert bench.passes_per_step(ndim, steps, chained) == pytest.approx(want)
 
 
-@pytest.mark.parametrize("n", [256, 512])
-@pytest.mark.parametrize("slot", ["x round trip", "y round trip", "z round trip", "double round trip [d N d]"])
+@pytest.mark.parametrize("n, slot", [(n, s) for n in (256, 512) for s in (
+    "x round trip", "y round trip", "z round trip", "double round trip [d N d]")] +
+    [(256, "double round trip [x x]")])
 def test_committed_traffic_is_close_to_the_algorithmic_bytes(slot, n):
     """The calibrated counter bytes of every kernel of the step sit within 1 %
     above one read and one write of the n³ complex FP32 field (no re-reads)."""
