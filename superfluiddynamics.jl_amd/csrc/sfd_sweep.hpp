@@ -137,6 +137,40 @@ __device__ __forceinline__ void nl_phase(cx<R>& v, R vpot, R beta, R dt, bool im
     }
 }
 
+// The real-time nonlinear phase of a lane's N1 FP32 sites.  When every site
+// of the wavefront has |th| <= pi/2 (the usual case: th = (V + beta|psi|^2)
+// dt is a small angle), each site takes one lifting rotation by -th, with
+// t = -tan(th/2) (|t| <= 1) and s = -sin th from the sine and cosine of th/2
+// (|th/2| <= pi/4: no 2 pi reduction), instead of two rotations by -th/2;
+// otherwise every site goes through nl_phase's reduced, halved rotations.
+template <int N1>
+__device__ __forceinline__ void nl_phase_lane(cx<float> (&v)[N1], const float (&vp)[N1], float beta, float dt)
+{
+    float th[N1];
+    float m = 0.0f;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+        const float a2 = v[j].re * v[j].re + v[j].im * v[j].im;
+        th[j] = (vp[j] + beta * a2) * dt;
+        m = fmaxf(m, fabsf(th[j]));
+    }
+    if (__builtin_amdgcn_ballot_w64(!(m <= 1.5707963f)) == 0) {
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            float sh, ch;
+            sincos_small(0.5f * th[j], &sh, &ch);
+            v[j] = lift(v[j], -sh * __builtin_amdgcn_rcpf(ch), -2.0f * sh * ch);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            float t, sn;
+            lift_half_neg(th[j], &t, &sn);
+            v[j] = lift(lift(v[j], t, sn), t, sn);
+        }
+    }
+}
+
 // Workgroup sums of acc[K] into partial[k * nblocks + block]; ACC adds them
 // to what the slot holds (a slot per workgroup: no atomics).
 template <typename R, int NT, int K, bool ACC = false>
@@ -606,9 +640,13 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
             else store_line<S, R, N1>(u, static_cast<cx<S>*>(a.save), gm);
         }
         const R beta = R(a.beta), dt = R(a.dt);
-        const bool imag = sizeof(R) == 4 ? IMAG : (flags & kFlagImag) != 0;
+        if constexpr (sizeof(R) == 4 && !IMAG) {
+            nl_phase_lane<N1>(v, vp, beta, dt);
+        } else {
+            const bool imag = sizeof(R) == 4 ? IMAG : (flags & kFlagImag) != 0;
 #pragma unroll
-        for (int j = 0; j < N1; ++j) nl_phase<R>(v[j], vp[j], beta, dt, imag);
+            for (int j = 0; j < N1; ++j) nl_phase<R>(v[j], vp[j], beta, dt, imag);
+        }
     }
     if (flags & kFlagNorm) {
         double s = 0.0;
@@ -1186,9 +1224,13 @@ __device__ __forceinline__ void round_trip_blu(cx<R> (&v)[N1], const R (&vp)[N1]
     if (flags & kFlagNL) {
         if (a.save) store_line_blu<S, R, N1, HN2>(v, static_cast<cx<S>*>(a.save), gm, tl, n);
         const R beta = R(a.beta), dt = R(a.dt);
-        const bool imag = sizeof(R) == 4 ? IMAG : (flags & kFlagImag) != 0;
+        if constexpr (sizeof(R) == 4 && !IMAG) {
+            nl_phase_lane<N1>(v, vp, beta, dt);
+        } else {
+            const bool imag = sizeof(R) == 4 ? IMAG : (flags & kFlagImag) != 0;
 #pragma unroll
-        for (int j = 0; j < N1; ++j) nl_phase<R>(v[j], vp[j], beta, dt, imag);
+            for (int j = 0; j < N1; ++j) nl_phase<R>(v[j], vp[j], beta, dt, imag);
+        }
     }
     if (flags & kFlagNorm) {
         double s = 0.0;
