@@ -337,6 +337,10 @@ int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, doubl
             if (imag) return fail(SFD_ERR_ARG, "no energy sums in an imaginary-time double round trip");
             a.epartial = epartial;
             a.xi = c->xi[axis];
+            // x: the two round trips of chained steps, the energy of the
+            // state between them; innermost axis: [d N d], the energy of the
+            // state it loads
+            if (axis == 0) return launch_sweep<P, kRoundTripTwiceEnergyX>(c, axis, a);
             return launch_sweep<P, kRoundTripTwiceEnergy>(c, axis, a);
         }
         if constexpr (sizeof(R) == 4)
@@ -425,23 +429,49 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
     bool pending = false;
     if (!(order == 2 && D >= 2 && (Om == 0.0 || D == 3))) fold = kFoldNone;
     if (fold == kFoldChain) {
-        if (!imag && Om == 0.0 && !ehist && nsteps >= 2 && !c->blu[0]) {
+        if (!imag && Om == 0.0 && nsteps >= 2 && !c->blu[0]) {
+            // With `ehist`, the energy of the state before step s comes out of
+            // the sweeps that first transform it: x from the opening x round
+            // trip (s = 0) or from the second round trip of [x x] (the state
+            // between steps s - 1 and s, with the pointwise sums), y from the
+            // y round trip that follows, z from the first round trip of
+            // [z N z]; one finalize per step.
+            FinalizeSets fs{};
+            auto record = [&](int ax, int64_t s) {
+                fs.partial[ax] = epart[ax];
+                fs.nb[ax] = c->last_nblocks;
+                fs.K[ax] = ax == 0 ? 4 : 1;
+                fs.out[ax] = ehist + 8 * s + (ax == 0 ? 0 : 3 + ax);
+            };
+            const int pw = ehist ? kFlagPointwise : 0;
             // x, y (first half of step 0)
-            for (int ax = 0; ax < D - 1; ++ax)
-                SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt, nullptr, nullptr, false, ws));
+            for (int ax = 0; ax < D - 1; ++ax) {
+                SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, ax == 0 ? pw : 0, beta, dt, nullptr,
+                                      epart[ax], false, ws));
+                if (ehist) record(ax, 0);
+            }
             for (int64_t s = 0; s < nsteps; ++s) {
-                SFD_TRY(round_trip<P>(c, psi, V, D - 1, tau, cD, Om, 0, nlflags, beta, dt, nullptr, nullptr, true,
-                                      ws));
+                SFD_TRY(round_trip<P>(c, psi, V, D - 1, tau, cD, Om, 0, nlflags, beta, dt, nullptr, epart[D - 1],
+                                      true, ws));
+                if (ehist) {
+                    record(D - 1, s);
+                    launch_finalize_sets(c->stream, fs);
+                    SFD_HIP(hipGetLastError());
+                }
                 // second half in reverse down to y
                 for (int ax = D - 2; ax >= 1; --ax)
                     SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt, nullptr, nullptr, false, ws));
                 if (s + 1 < nsteps) {
-                    // x of step s and x of step s + 1: no nonlinear phase, no potential
-                    SFD_TRY(round_trip<P>(c, psi, nullptr, 0, tau, cD, Om, 0, 0, beta, dt, nullptr, nullptr, true,
-                                          ws));
-                    for (int ax = 1; ax < D - 1; ++ax)
-                        SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt, nullptr, nullptr, false,
+                    // x of step s and x of step s + 1: no nonlinear phase (no
+                    // potential, unless the energy sums need it)
+                    SFD_TRY(round_trip<P>(c, psi, ehist ? V : nullptr, 0, tau, cD, Om, 0, pw, beta, dt, nullptr,
+                                          epart[0], true, ws));
+                    if (ehist) record(0, s + 1);
+                    for (int ax = 1; ax < D - 1; ++ax) {
+                        SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt, nullptr, epart[ax], false,
                                               ws));
+                        if (ehist) record(ax, s + 1);
+                    }
                 } else {
                     SFD_TRY(round_trip<P>(c, psi, V, 0, tau, cD, Om, 0, 0, beta, dt, nullptr, nullptr, false, ws));
                 }
@@ -562,9 +592,12 @@ int energy_impl(sfd_ctx* c, const void* psi, const void* V, double cD, double be
 // with Omega = 0 the energy comes out of each step's first-half sweeps
 // (split_impl with a history): the y and z sums are taken after the x (and
 // y) flow, which is unitary along its axis and commutes with the y/z
-// transforms, so the sums are the state's own.  With rotation the x flow's
-// phase depends on y, and in imaginary time the flows damp; each step is
-// then preceded by a separate energy evaluation.  energies: host, 5 per step.
+// transforms, so the sums are the state's own.  The steps are chained
+// (kFoldChain): the x sums of the state between two steps come out of the
+// second round trip of the [x x] pass that joins them.  With rotation the x
+// flow's phase depends on y, and in imaginary time the flows damp; each step
+// is then preceded by a separate energy evaluation.  energies: host, 5 per
+// step.
 template <typename P>
 int split_energy_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt,
                       int64_t nsteps, int order, int mode, double* energies)
@@ -581,7 +614,7 @@ int split_energy_impl(sfd_ctx* c, void* psi, const void* V, double cD, double be
     for (int64_t s0 = 0; s0 < nsteps; s0 += kChunk) {
         const int64_t m = std::min(kChunk, nsteps - s0);
         SFD_TRY(ensure_buf(c, &c->ehist, &c->ehist_cap, 8 * m));
-        SFD_TRY(split_impl<P>(c, psi, V, cD, beta, Om, dt, m, order, mode, c->ehist));
+        SFD_TRY(split_impl<P>(c, psi, V, cD, beta, Om, dt, m, order, mode, c->ehist, kFoldChain));
         h.resize(8 * (size_t)m);
         SFD_HIP(hipMemcpyAsync(h.data(), c->ehist, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         SFD_HIP(hipStreamSynchronize(c->stream));

@@ -35,6 +35,8 @@ enum SweepMode : int {
     kRoundTripTwice = 7,         // round trip, nonlinear phase, round trip (same multiplier)
     kRoundTripTwiceImag = 8,     // kRoundTripTwice for imaginary time (FP32 sweeps)
     kRoundTripTwiceEnergy = 9,   // kRoundTripTwice whose first round trip sums the energy
+    kRoundTripTwiceEnergyX = 10, // two round trips (x axis of chained steps) whose second round trip sums
+                                 // the energy of the state between them
 };
 
 constexpr bool is_round_trip(int mode)
@@ -44,7 +46,8 @@ constexpr bool is_round_trip(int mode)
 }
 constexpr bool rt_twice(int mode)
 {
-    return mode == kRoundTripTwice || mode == kRoundTripTwiceImag || mode == kRoundTripTwiceEnergy;
+    return mode == kRoundTripTwice || mode == kRoundTripTwiceImag || mode == kRoundTripTwiceEnergy ||
+           mode == kRoundTripTwiceEnergyX;
 }
 constexpr bool rt_imag(int mode)
 {
@@ -52,7 +55,8 @@ constexpr bool rt_imag(int mode)
 }
 constexpr bool rt_energy(int mode)
 {
-    return mode == kRoundTripEnergy || mode == kRoundTripImagEnergy || mode == kRoundTripTwiceEnergy;
+    return mode == kRoundTripEnergy || mode == kRoundTripImagEnergy || mode == kRoundTripTwiceEnergy ||
+           mode == kRoundTripTwiceEnergyX;
 }
 
 enum SweepFlags : int {
@@ -789,7 +793,13 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
         // row: no rotation term on this axis) is the same for both.
         // kRoundTripTwiceEnergy: the first round trip also sums the energy
         // of the loaded state (spectral sum only: this is never the x axis).
+        // kRoundTripTwiceEnergyX: the two x round trips of chained steps,
+        // the second summing the energy of the state between them (the
+        // state at the step boundary; spectral x sum and, with
+        // kFlagPointwise, the pointwise sums, V read for them).
         constexpr bool EN = rt_energy(MODE);
+        constexpr bool EN1 = MODE == kRoundTripTwiceEnergy;
+        constexpr bool EN2 = MODE == kRoundTripTwiceEnergyX;
         constexpr bool CT = centered_ok<R, N1, N2, H>;
         const unsigned lane0 = CT ? ct_lane0<N, N2>(gm, t) : 0u;
         if constexpr (CT) load_line_ct<S, R, N1, N2>(v, psi, gm, lane0);
@@ -797,7 +807,7 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
         if constexpr (ZV) {
 #pragma unroll
             for (int j = 0; j < N1; ++j) vp[j] = R(0);
-        } else if (flags & kFlagNL) {
+        } else if ((flags & kFlagNL) || (EN2 && (flags & kFlagPointwise))) {
             if constexpr (CT) load_pot_ct<S, R, N1, N2>(vp, static_cast<const S*>(a.V), gm, lane0);
             else load_pot<S, R, N1>(vp, static_cast<const S*>(a.V), gm);
         }
@@ -818,16 +828,23 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
             __shared__ cx<R> tabs[NTW + N];
             stage_tables<R, M, H, Sh::NT, CT>(tabs, a, tid);
             // FP32 real time: the first round trip leaves N psi (no 1/N),
-            // the second applies 1/N^2 (see round_trip_tile's SC)
-            constexpr int SC1 = (sizeof(R) == 4 && !rt_imag(MODE)) ? 0 : 1;
+            // the second applies 1/N^2 (see round_trip_tile's SC); not when
+            // the second sums the energy of the state between them
+            constexpr int SC1 = (sizeof(R) == 4 && !rt_imag(MODE) && !EN2) ? 0 : 1;
             constexpr int SC2 = SC1 == 0 ? 2 : 1;
             if constexpr (SC1 == 0) a1.beta = a.beta / ((double)N * (double)N);
-            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN, H, CT, SC1>(
+            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN1, H, CT, SC1>(
                 v, vp, gm, a1, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc, eacc,
                 lane0);
-            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H, CT, SC2>(
+            if constexpr (EN2 && sizeof(S) != sizeof(R)) {
+                // the state between the steps as a step would store it: the
+                // energy sums and the next step see the field's precision
+#pragma unroll
+                for (int j = 0; j < N1; ++j) v[j] = widen<R>(narrow<S>(v[j]));
+            }
+            round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), EN2, H, CT, SC2>(
                 v, vp, gm, a2, lds, tabs + t, tabs + M + t, tabs + 2 * M + t, tabs + NTW, c, gh, t, h, nacc,
-                nullptr, lane0);
+                eacc, lane0);
         } else {
             const cx<R>* ph = row_phase<R>(a, gm);
             round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H>(v, vp, gm, a1, lds, twA, twB, twC,
@@ -840,7 +857,10 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
             block_partials<R, Sh::NT, 1>(acc1, a.partial, nblocks, tile);
         }
         if constexpr (EN) {
-            if (!gm.active) eacc[0] = 0.0;
+            if (!gm.active) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) eacc[k] = 0.0;
+            }
             block_partials<R, Sh::NT, 4>(eacc, a.epartial, nblocks, tile);
         }
         if constexpr (CT) store_line_ct<S, R, N1, N2>(v, psi, gm, lane0);

@@ -158,9 +158,24 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
                         (k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W, V0::H, V0::T, true>), grid,
                         block, 0, s, a);
                     return true;
+                } else if constexpr (MODE == kRoundTripTwiceEnergyX) {
+                    // with the energy sums of the state between them (V for the
+                    // pointwise sums)
+                    if (a.flags & (kFlagNL | kFlagScale | kFlagNorm)) return false;
+                    if (a.V)
+                        hipLaunchKernelGGL(
+                            (k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W, V0::H, V0::T, false>),
+                            grid, block, 0, s, a);
+                    else
+                        hipLaunchKernelGGL(
+                            (k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W, V0::H, V0::T, true>), grid,
+                            block, 0, s, a);
+                    return true;
                 } else {
                     return false;
                 }
+            } else if constexpr (MODE == kRoundTripTwiceEnergyX) {
+                return false;  // x axis only
             } else if constexpr (rt_twice(MODE) && !rt_energy(MODE)) {
                 if (!a.V)
                     hipLaunchKernelGGL(

@@ -458,8 +458,12 @@ def solve(n: _SplitBase, istart: int = 1, verbose: bool = False, writers=()):
     (energy, timeStep!); returns the history of (nstep, total, EΩ, EΔ, Eβ, E).
     ``writers`` (``sfdhip.io.WriterSave`` objects; the reference's VTK writer
     is not built) save "res-{istep}.h5" at istart and after every iteration
-    it with it % freqbckp == 0.  The iterations between two saves run in one
-    sfd_split_step_energy call; the printed lines follow it."""
+    it with it % freqbckp == 0.  The iterations between two backup points run
+    in one sfd_split_step_energy call, with or without writers, so the
+    history does not depend on the writers, and a run resumed from a backup
+    with the same freqbckp repeats the original bit for bit (chained steps
+    within a call order the commuting flows by the call's length); the
+    printed lines follow each call."""
     from . import io as _io
     writers = list(writers)
 
@@ -472,7 +476,7 @@ def solve(n: _SplitBase, istart: int = 1, verbose: bool = False, writers=()):
     total = 0
     it, last = istart, istart + n.niter
     while it <= last:
-        stop = min(last, it + (-it) % n.freqbckp) if writers else last
+        stop = min(last, it + (-it) % n.freqbckp)
         hist = steps_with_energy(n, stop - it + 1)
         for j in range(stop - it + 1):
             EΩ, EΔ, Eβ, E, μ = (float(x) for x in hist[j])

@@ -55,12 +55,13 @@ def test_solve_writes_backups_and_resumes(gpu, tmp_path, monkeypatch):
     assert len(res) == 21
     assert sorted(os.listdir(tmp_path)) == sorted(f"res-{i}.h5" for i in (1, 4, 8, 12, 16, 20))
     final = m.f.to_numpy()
-    # the same history without writers (one fused call)
+    # the same history without writers (the same calls between backup points)
     m2 = _model(gpu, psi, bounds, np.float64, 20, 4, V)
     res2 = gpu.solve(m2)
     assert np.array_equal(np.array(res), np.array(res2))
-    # resume: the state after iteration 12, then iterations 13..21
-    m3 = _model(gpu, np.zeros_like(psi), bounds, np.float64, 8, 1000, V)
+    # resume: the state after iteration 12, then iterations 13..21 with the
+    # same backup frequency, hence the same calls
+    m3 = _model(gpu, np.zeros_like(psi), bounds, np.float64, 8, 4, V)
     gpu.read(gpu.WriterSave(m3.f), prefix="res", istep=12)
     res3 = gpu.solve(m3, istart=13)
     assert np.allclose(np.array(res3)[:, 2:], np.array(res)[12:, 2:], rtol=0, atol=1e-13 * abs(res[0][5]))
