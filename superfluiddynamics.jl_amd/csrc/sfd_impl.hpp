@@ -324,7 +324,7 @@ int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, doubl
     a.dt = dt;
     a.scale = ws ? ws->scal : c->scal;
     a.save = save;
-    if (flags & kFlagNorm) {
+    if (flags & (kFlagNorm | kFlagNormMid)) {
         if (ws) {
             a.partial = ws->partial;
         } else {
@@ -408,10 +408,13 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
     const int D = c->ndim;
     const double tau = order == 2 ? dt * 0.5 : dt;
     const int nlflags = kFlagNL | (imag ? kFlagImag : 0);
-    // Reduction workspace for the per-step renormalisation (its last sweep
-    // is on axis D - 2 when folded, else on the last axis).
-    if (imag && !ws)
-        SFD_TRY(ensure_partial(c, 5 * std::max(sweep_nblocks(c, D >= 2 ? D - 2 : 0), sweep_nblocks(c, D - 1))));
+    // Reduction workspace for the per-step renormalisation (its sweep is on
+    // axis D - 2 when folded, on the last axis when not, on x when chained).
+    if (imag && !ws) {
+        long long nb = 0;
+        for (int ax = 0; ax < D; ++ax) nb = std::max(nb, sweep_nblocks(c, ax));
+        SFD_TRY(ensure_partial(c, 5 * nb));
+    }
     double* scal = ws ? ws->scal : c->scal;
     double* npart = ws ? ws->partial : c->partial;
     // Energy partials: one region per axis, all finalized after the step's
@@ -429,13 +432,18 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
     bool pending = false;
     if (!(order == 2 && D >= 2 && (Om == 0.0 || D == 3))) fold = kFoldNone;
     if (fold == kFoldChain) {
-        if (!imag && Om == 0.0 && nsteps >= 2 && !c->blu[0]) {
+        if (Om == 0.0 && nsteps >= 2 && !c->blu[0] && !(imag && ehist)) {
             // With `ehist`, the energy of the state before step s comes out of
             // the sweeps that first transform it: x from the opening x round
             // trip (s = 0) or from the second round trip of [x x] (the state
             // between steps s - 1 and s, with the pointwise sums), y from the
             // y round trip that follows, z from the first round trip of
             // [z N z]; one finalize per step.
+            // In imaginary time [x x] takes the norm partials of the state
+            // between its round trips (the end of step s, kFlagNormMid); the
+            // renormalisation they give is applied by the next sweep (a
+            // scalar commutes with the x flow of step s + 1), before the next
+            // nonlinear phase.
             FinalizeSets fs{};
             auto record = [&](int ax, int64_t s) {
                 fs.partial[ax] = epart[ax];
@@ -443,16 +451,28 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
                 fs.K[ax] = ax == 0 ? 4 : 1;
                 fs.out[ax] = ehist + 8 * s + (ax == 0 ? 0 : 3 + ax);
             };
+            auto renorm = [&]() -> int {
+                launch_finalize(c->stream, npart, c->last_nblocks, 1, scal + 1, c->dvol(), scal);
+                SFD_HIP(hipGetLastError());
+                pending = true;
+                return SFD_OK;
+            };
             const int pw = ehist ? kFlagPointwise : 0;
+            // first sweep after a renormalisation applies it
+            auto take_scale = [&]() -> int {
+                const int f = pending ? kFlagScale : 0;
+                pending = false;
+                return f;
+            };
             // x, y (first half of step 0)
             for (int ax = 0; ax < D - 1; ++ax) {
-                SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, ax == 0 ? pw : 0, beta, dt, nullptr,
+                SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, ax == 0 ? pw : 0, beta, dt, nullptr,
                                       epart[ax], false, ws));
                 if (ehist) record(ax, 0);
             }
             for (int64_t s = 0; s < nsteps; ++s) {
-                SFD_TRY(round_trip<P>(c, psi, V, D - 1, tau, cD, Om, 0, nlflags, beta, dt, nullptr, epart[D - 1],
-                                      true, ws));
+                SFD_TRY(round_trip<P>(c, psi, V, D - 1, tau, cD, Om, imag, nlflags | (D == 2 ? take_scale() : 0),
+                                      beta, dt, nullptr, epart[D - 1], true, ws));
                 if (ehist) {
                     record(D - 1, s);
                     launch_finalize_sets(c->stream, fs);
@@ -460,25 +480,29 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
                 }
                 // second half in reverse down to y
                 for (int ax = D - 2; ax >= 1; --ax)
-                    SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt, nullptr, nullptr, false, ws));
+                    SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, 0, beta, dt, nullptr, nullptr, false, ws));
                 if (s + 1 < nsteps) {
                     // x of step s and x of step s + 1: no nonlinear phase (no
                     // potential, unless the energy sums need it)
-                    SFD_TRY(round_trip<P>(c, psi, ehist ? V : nullptr, 0, tau, cD, Om, 0, pw, beta, dt, nullptr,
-                                          epart[0], true, ws));
+                    SFD_TRY(round_trip<P>(c, psi, ehist ? V : nullptr, 0, tau, cD, Om, imag,
+                                          pw | (imag ? kFlagNormMid : 0), beta, dt, nullptr, epart[0], true, ws));
                     if (ehist) record(0, s + 1);
+                    if (imag) SFD_TRY(renorm());
                     for (int ax = 1; ax < D - 1; ++ax) {
-                        SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, 0, 0, beta, dt, nullptr, epart[ax], false,
-                                              ws));
+                        SFD_TRY(round_trip<P>(c, psi, V, ax, tau, cD, Om, imag, ax == 1 ? take_scale() : 0, beta, dt,
+                                              nullptr, epart[ax], false, ws));
                         if (ehist) record(ax, s + 1);
                     }
                 } else {
-                    SFD_TRY(round_trip<P>(c, psi, V, 0, tau, cD, Om, 0, 0, beta, dt, nullptr, nullptr, false, ws));
+                    SFD_TRY(round_trip<P>(c, psi, V, 0, tau, cD, Om, imag, imag ? kFlagNorm : 0, beta, dt, nullptr,
+                                          nullptr, false, ws));
+                    if (imag) SFD_TRY(renorm());
                 }
             }
-            return SFD_OK;
+            nsteps = 0;  // done: only the closing renormalisation below remains
+        } else {
+            fold = kFoldInner;
         }
-        fold = kFoldInner;
     }
     for (int64_t s = 0; s < nsteps && fold; ++s) {
         FinalizeSets fs{};

@@ -64,7 +64,8 @@ enum SweepFlags : int {
     kFlagImag = 2,      // imaginary time: NL factor exp(-(V + beta|psi|^2) dt)
     kFlagNorm = 4,      // epilogue: per-workgroup sum |psi|^2 into partial[]
     kFlagScale = 8,     // prologue: psi *= *scale (pending renormalisation)
-    kFlagPointwise = 16 // energy sums: also sum |psi|^2, V|psi|^2, |psi|^4
+    kFlagPointwise = 16, // energy sums: also sum |psi|^2, V|psi|^2, |psi|^4
+    kFlagNormMid = 32    // double round trip: the norm partials of the state between its round trips
 };
 
 struct SweepArgs {
@@ -813,10 +814,14 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
         }
         double nacc = 0.0;
         double eacc[4] = {0, 0, 0, 0};
+        // the norm partials after the second round trip (kFlagNorm) or, in
+        // the x pass of chained imaginary-time steps, after the first
+        // (kFlagNormMid: the state that ends a step, before its
+        // renormalisation, which the next sweep applies)
         SweepArgs a1 = a;
-        a1.flags = flags & ~kFlagNorm;
+        a1.flags = (flags & ~(kFlagNorm | kFlagNormMid)) | ((flags & kFlagNormMid) ? kFlagNorm : 0);
         SweepArgs a2 = a;
-        a2.flags = flags & ~(kFlagNL | kFlagScale);
+        a2.flags = flags & ~(kFlagNL | kFlagScale | kFlagNormMid | ((flags & kFlagNormMid) ? kFlagNorm : 0));
         a2.save = nullptr;
         // FP32 (and energy) sweeps read the tables from LDS, staged once
         // per workgroup: 104 instead of 126 VGPRs,
@@ -852,7 +857,7 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
             round_trip_tile<S, R, N1, N2, TS, H * G, rt_imag(MODE), false, H>(v, vp, gm, a2, lds, twA, twB, twC,
                                                                               ph, c, gh, t, h, nacc);
         }
-        if (flags & kFlagNorm) {
+        if (flags & (kFlagNorm | kFlagNormMid)) {
             double acc1[1] = {nacc};
             block_partials<R, Sh::NT, 1>(acc1, a.partial, nblocks, tile);
         }

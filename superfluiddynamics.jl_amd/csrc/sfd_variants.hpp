@@ -152,7 +152,7 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
             if constexpr (rt_twice(MODE) && V0::TS == 1) {
                 // the x axis runs only the double round trip [x x] of chained
                 // Strang steps: real time, no nonlinear phase, no potential
-                if constexpr (MODE == kRoundTripTwice) {
+                if constexpr (MODE == kRoundTripTwice || MODE == kRoundTripTwiceImag) {
                     if (a.V || (a.flags & (kFlagNL | kFlagScale | kFlagNorm))) return false;
                     hipLaunchKernelGGL(
                         (k_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, MODE, V0::W, V0::H, V0::T, true>), grid,
