@@ -30,6 +30,10 @@ CASES = [
     ((2048, 8), ((-10, 10), (-1, 1)), 0.0, True),
     ((8, 2048), ((-1, 1), (-10, 10)), 0.2, False),
     ((24, 2048), ((-2, 2), (-10, 10)), 0.0, True),
+    # power-of-two x with Bluestein y / z at Omega = 0: chained steps ([x x])
+    # around Bluestein sweeps
+    ((64, 48), ((-6, 6), (-5, 5)), 0.0, True),
+    ((32, 24, 20), ((-4, 4), (-3, 3), (-3, 3)), 0.0, True),
 ]
 
 
@@ -115,7 +119,7 @@ def test_energy_any_size(gpu, orc, case):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32x"])
-@pytest.mark.parametrize("case", [CASES[0], CASES[3]], ids=_ids)
+@pytest.mark.parametrize("case", [CASES[0], CASES[3], CASES[-2], CASES[-1]], ids=_ids)
 def test_imag_time_any_size(gpu, orc, case, prec):
     shape, bounds, Om, _ = case
     psi = smooth_state(shape, bounds, seed=9)
