@@ -235,11 +235,14 @@ int sfd_batch_imag_run(sfd_batch* b, void* const* psi, const void* const* V, con
 
 /* Autotuner of a batch: for every shape group and axis, times each compiled
  * launch variant of the batched sweep (identity round trips on the batch's
- * own scratch, the fields are not touched) and keeps the fastest, like
- * sfd_autotune for one context.  times_ms (may be NULL) receives at least
- * 24*ninst floats: [24 g + 8 axis + variant] = ms per launch, -1 where there
- * is none, g in the batch's internal group order.  No reference counterpart
- * (FFTW plan flags, Plans.jl:847-848, play this role there). */
+ * own scratch, the fields are not touched; the double round trip on x and on
+ * the folded axis, as a solve runs them) and keeps the fastest, like
+ * sfd_autotune for one context.  An FP32 or F32X batch also tunes the FP64
+ * level its instances finish on (SFD_BATCH_ESCALATE), whose buffers it then
+ * allocates.  times_ms (may be NULL) receives at least 24*ninst floats of
+ * the batch's own precision: [24 g + 8 axis + variant] = ms per launch, -1
+ * where there is none, g in the batch's internal group order.  No reference
+ * counterpart (FFTW plan flags, Plans.jl:847-848, play this role there). */
 int sfd_batch_autotune(sfd_batch* b, float* times_ms);
 
 /* Adjoint of the real-time Strang split (no counterpart in the reference):

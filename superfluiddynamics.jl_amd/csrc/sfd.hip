@@ -408,9 +408,9 @@ int batch_setup(sfd_batch* b)
 }
 
 // Enqueue k imaginary-time Strang steps of every (group, level) set in
-// `mask` (bit 2 g + level); the last step is a convergence check, after
-// which level-0 instances at their FP32 floor move to FP64 when
-// `escalate`.  Each size group runs on its own stream (forked from and
+// `mask` (bit 2 g + level), chained where the group chains; the last step
+// is a convergence check, after which level-0 instances at their FP32
+// floor move to FP64 when `escalate`.  Each size group runs on its own stream (forked from and
 // joined back to the batch stream), so the groups' small launches overlap.
 int batch_enqueue(sfd_batch* b, unsigned long long mask, int k, double tol, bool escalate)
 {
@@ -420,38 +420,62 @@ int batch_enqueue(sfd_batch* b, unsigned long long mask, int k, double tol, bool
         if (!act[0] && !act[1]) continue;
         hipStream_t st = b->gstream[g];
         SFD_HIP(hipStreamWaitEvent(st, b->fork, 0));
-        for (int s = 0; s < k; ++s) {
-            const bool chk = s == k - 1;
-            for (int L = 0; L < sfd_batch::kLevels; ++L)
-                if (act[L])
-                    SFD_TRY(ops_for(b->lv[L].prec).batch_level_step(b, L, (int)g, st, chk, tol, L == 0 && escalate));
-            if (chk && escalate && act[0]) SFD_TRY(ops_for(b->prec).batch_escalate(b, (int)g, st));
-        }
+        // the two levels advance disjoint sets of instances (done flags), so
+        // each runs its k steps in turn; escalation follows both checks
+        for (int L = 0; L < sfd_batch::kLevels; ++L)
+            if (act[L]) SFD_TRY(ops_for(b->lv[L].prec).batch_level_steps(b, L, (int)g, st, k, tol, L == 0 && escalate));
+        if (escalate && act[0]) SFD_TRY(ops_for(b->prec).batch_escalate(b, (int)g, st));
         SFD_HIP(hipEventRecord(b->gdone[g], st));
         SFD_HIP(hipStreamWaitEvent(b->stream, b->gdone[g], 0));
     }
     return SFD_OK;
 }
 
-// The step of each group for this solve: x, [y N y], x in 2D when no
-// instance of the group rotates (the folded Strang step), x, y + N, x, y in
-// 2D with rotation (K_x and K_y do not commute), x, y, [z N z], x, y in 3D.
-// The first sweep applies the pending renormalisation, the last one sums
-// the norm.
+// The step of each group for this solve, and its chained form.  The first
+// sweep of a step applies the pending renormalisation, the last one sums
+// the norm, and a renormalisation (k_batch_finalize) ends it.
+// - With rotation (K_x and K_y do not commute): x, y + N, x, y in 2D;
+//   x, y, [z N z], x, y in 3D.  Not chained.
+// - Without: x, [y N y], x in 2D (the folded Strang step); x, y, [z N z],
+//   y, x in 3D.  Chained steps share their x flows: x, [y N y], [x x],
+//   [y N y], ..., x in 2D and x, y, [z N z], y, [x x], y, ..., y, x in 3D,
+//   where [x x] sums the norm of the state between its round trips (the one
+//   that ends a step) and the sweep after it applies the renormalisation.
 void batch_kinds(sfd_batch* b, const double* Omega)
 {
     using K = sfd_batch::Kind;
+    constexpr int F = sfd_batch::kFinalize;
     for (auto& G : b->groups) {
         bool rot = false;
         for (int i = G.first; i < G.first + G.count; ++i)
             rot = rot || (Omega && Omega[b->inst[i].user] != 0.0);
-        if (G.ndim == 3)
+        G.chain = !rot;
+        G.open.clear();
+        G.body.clear();
+        G.close.clear();
+        if (G.ndim == 3 && rot) {
             G.kinds = {K{0, kFlagScale, false}, K{1, 0, false}, K{2, kFlagNL, true}, K{0, 0, false},
                        K{1, kFlagNorm, false}};
-        else if (!rot)
-            G.kinds = {K{0, kFlagScale, false}, K{1, kFlagNL, true}, K{0, kFlagNorm, false}};
-        else
+            G.step = {0, 1, 2, 3, 4, F};
+        } else if (G.ndim == 3) {
+            G.kinds = {K{0, kFlagScale, false}, K{1, 0, false},          K{2, kFlagNL, true},
+                       K{1, 0, false},          K{0, kFlagNorm, false},  K{0, kFlagNormMid, true},
+                       K{1, kFlagScale, false}};
+            G.step = {0, 1, 2, 3, 4, F};
+            G.open = {0, 1, 2, 3};
+            G.body = {5, F, 6, 2, 3};
+            G.close = {4, F};
+        } else if (rot) {
             G.kinds = {K{0, kFlagScale, false}, K{1, kFlagNL, false}, K{0, 0, false}, K{1, kFlagNorm, false}};
+            G.step = {0, 1, 2, 3, F};
+        } else {
+            G.kinds = {K{0, kFlagScale, false}, K{1, kFlagNL, true}, K{0, kFlagNorm, false},
+                       K{0, kFlagNormMid, true}, K{1, kFlagNL | kFlagScale, true}};
+            G.step = {0, 1, 2, F};
+            G.open = {0, 1};
+            G.body = {3, F, 4};
+            G.close = {2, F};
+        }
     }
 }
 
@@ -866,7 +890,14 @@ int sfd_batch_autotune(sfd_batch* b, float* times_ms)
     g_last_error.clear();
     if (!b) return fail(SFD_ERR_NULL, "sfd_batch_autotune: null batch");
     SFD_HIP(hipSetDevice(b->device));
-    return ops_for(b->prec).batch_autotune(b, times_ms);
+    SFD_TRY(ops_for(b->prec).batch_autotune(b, 0, times_ms));
+    if (b->prec == SFD_F64) return SFD_OK;
+    // the FP64 level that escalated instances finish on has its own variants
+    if (!b->lv[1].prec) {
+        b->lv[1].prec = SFD_F64;
+        SFD_TRY(ops_for(SFD_F64).batch_level_setup(b, 1));
+    }
+    return ops_for(SFD_F64).batch_autotune(b, 1, nullptr);
 }
 
 int sfd_batch_destroy(sfd_batch* b)

@@ -1099,7 +1099,7 @@ int batch_level_bind(sfd_batch* b, int L, void* const* psi, const void* const* V
             a.beta = beta[in.user];
             a.dt = dt;
             a.flags = kFlagImag | kd.flags;
-            if (kd.flags & kFlagNorm) {
+            if (kd.flags & (kFlagNorm | kFlagNormMid)) {  // every norm sweep of a group is on one axis
                 a.partial = hi[i].partial;
                 norm_axis = ax;
             }
@@ -1115,24 +1115,21 @@ int batch_level_bind(sfd_batch* b, int L, void* const* psi, const void* const* V
     return SFD_OK;
 }
 
-// One imaginary-time Strang step of the unfinished instances of group g at
-// level L on stream st: the group's sweeps, the per-instance renormalisation
-// factor and, on a check step, the snapshot before and ||Delta psi|| after
-// (with the escalation policy of k_batch_diff_finalize when `escalate`).
+// The sweeps (and renormalisations, sfd_batch::kFinalize) of one sequence
+// of group g at level L on stream st, over its unfinished instances.
 template <typename P>
-int batch_level_step(sfd_batch* b, int L, int g, hipStream_t st, bool check, double tol, bool escalate)
+int batch_level_run(sfd_batch* b, int L, int g, hipStream_t st, const std::vector<int>& seq)
 {
-    using S = typename P::S;
     const auto& G = b->groups[g];
     const auto& lv = b->lv[L];
     const int ni = b->ninst, f = G.first;
-    constexpr int KB = sfd_batch::kDiffBlocks;
-    if (check) {
-        hipLaunchKernelGGL(k_batch_snapshot<S>, dim3(KB, G.count), dim3(256), 0, st, lv.d_inst + f, b->d_scale + f,
-                           b->d_snap_scale + f, lv.d_done + f);
-        SFD_HIP(hipGetLastError());
-    }
-    for (size_t k = 0; k < G.kinds.size(); ++k) {
+    for (const int k : seq) {
+        if (k == sfd_batch::kFinalize) {
+            hipLaunchKernelGGL(k_batch_finalize, dim3(G.count), dim3(256), 0, st, lv.d_inst + f, b->d_scale + f,
+                               lv.d_done + f, b->d_steps + f);
+            SFD_HIP(hipGetLastError());
+            continue;
+        }
         const int ax = G.kinds[k].axis;
         const VariantInfo& vi = G.vi[L][ax];
         dim3 grid((unsigned)G.ntiles[L][ax], 1, (unsigned)G.count);
@@ -1142,18 +1139,45 @@ int batch_level_step(sfd_batch* b, int L, int g, hipStream_t st, bool check, dou
             return fail(SFD_ERR_SHAPE, "batch: no kernel for line length %d", G.n[ax]);
         SFD_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_batch_finalize, dim3(G.count), dim3(256), 0, st, lv.d_inst + f, b->d_scale + f,
-                       lv.d_done + f, b->d_steps + f);
-    SFD_HIP(hipGetLastError());
-    if (check) {
-        hipLaunchKernelGGL(k_batch_diff<S>, dim3(KB, G.count), dim3(256), 0, st, lv.d_inst + f, b->d_scale + f,
-                           b->d_snap_scale + f, lv.d_done + f, b->d_dpart + (size_t)f * KB);
-        SFD_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_batch_diff_finalize, dim3(G.count), dim3(256), 0, st, b->d_dpart + (size_t)f * KB, KB,
-                           lv.d_inst + f, b->d_dpsi + f, b->d_dprev + f, lv.d_done + f, tol, b->d_esc + f,
-                           escalate ? 1 : 0);
-        SFD_HIP(hipGetLastError());
+    return SFD_OK;
+}
+
+// k imaginary-time Strang steps of the unfinished instances of group g at
+// level L on stream st, the last one a convergence check: the snapshot
+// before it and ||Delta psi|| after it (with the escalation policy of
+// k_batch_diff_finalize when `escalate`).  Each step ends with the
+// per-instance renormalisation factor, which the next sweep applies.  A
+// group that chains runs the k - 1 steps before the check as one chain
+// (open, body k - 2 times, close): the x flow that ends a step and the one
+// that starts the next are one [x x] pass, whose first round trip takes the
+// norm of the state that ends the step; the check step is never chained, as
+// the snapshot needs the state before it in memory.
+template <typename P>
+int batch_level_steps(sfd_batch* b, int L, int g, hipStream_t st, int k, double tol, bool escalate)
+{
+    using S = typename P::S;
+    const auto& G = b->groups[g];
+    const auto& lv = b->lv[L];
+    const int f = G.first;
+    constexpr int KB = sfd_batch::kDiffBlocks;
+    if (G.chain && k >= 3) {
+        SFD_TRY(batch_level_run<P>(b, L, g, st, G.open));
+        for (int s = 1; s < k - 1; ++s) SFD_TRY(batch_level_run<P>(b, L, g, st, G.body));
+        SFD_TRY(batch_level_run<P>(b, L, g, st, G.close));
+    } else {
+        for (int s = 0; s < k - 1; ++s) SFD_TRY(batch_level_run<P>(b, L, g, st, G.step));
     }
+    hipLaunchKernelGGL(k_batch_snapshot<S>, dim3(KB, G.count), dim3(256), 0, st, lv.d_inst + f, b->d_scale + f,
+                       b->d_snap_scale + f, lv.d_done + f);
+    SFD_HIP(hipGetLastError());
+    SFD_TRY(batch_level_run<P>(b, L, g, st, G.step));
+    hipLaunchKernelGGL(k_batch_diff<S>, dim3(KB, G.count), dim3(256), 0, st, lv.d_inst + f, b->d_scale + f,
+                       b->d_snap_scale + f, lv.d_done + f, b->d_dpart + (size_t)f * KB);
+    SFD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_batch_diff_finalize, dim3(G.count), dim3(256), 0, st, b->d_dpart + (size_t)f * KB, KB,
+                       lv.d_inst + f, b->d_dpsi + f, b->d_dprev + f, lv.d_done + f, tol, b->d_esc + f,
+                       escalate ? 1 : 0);
+    SFD_HIP(hipGetLastError());
     return SFD_OK;
 }
 
@@ -1210,19 +1234,20 @@ int batch_finish(sfd_batch* b)
     return SFD_OK;
 }
 
-// Batch autotuner (level 0): for every size group and axis with more than
+// Batch autotuner of level L: for every size group and axis with more than
 // one compiled variant, time the identity round trip (tau = 0) of
 // k_batch_sweep over the group's instances on the level's snapshot buffer
 // filled with unit-modulus pseudo-random data (the fields are not touched),
-// as the double round trip on the axis whose flows the step folds (y in 2D,
-// z in 3D); 2 warm-up launches, then the best of 3 batches of 8, and keep
-// the fastest.  times[24 g + 8 ax + v] = ms per launch (-1 if none).
+// as the double round trip on the axes that run as one in a solve: the axis
+// whose flows the step folds (y in 2D, z in 3D) and x ([x x] of chained
+// steps); 2 warm-up launches, then the best of 3 batches of 8, and keep the
+// fastest.  times[24 g + 8 ax + v] = ms per launch (-1 if none).
 template <typename P>
-int batch_autotune(sfd_batch* b, float* times)
+int batch_autotune(sfd_batch* b, int L, float* times)
 {
     using R = typename P::R;
     using S = typename P::S;
-    auto& lv = b->lv[0];
+    auto& lv = b->lv[L];
     const int ni = b->ninst;
     if (times)
         for (int i = 0; i < 24 * ni; ++i) times[i] = -1.0f;
@@ -1262,7 +1287,7 @@ int batch_autotune(sfd_batch* b, float* times)
                 float best = 1e30f;
                 int best_v = 0;
                 for (int v = 0; v < nv && v < 8; ++v) {
-                    SFD_TRY(batch_level_variant<P>(b, 0, (int)gi, ax, v));
+                    SFD_TRY(batch_level_variant<P>(b, L, (int)gi, ax, v));
                     desc.assign(G.count, SweepArgs{});
                     for (int k = 0; k < G.count; ++k) {
                         const auto& in = b->inst[G.first + k];
@@ -1270,18 +1295,18 @@ int batch_autotune(sfd_batch* b, float* times)
                         std::memset(&a, 0, sizeof a);
                         a.psi = static_cast<char*>(lv.d_snap) + (size_t)in.soff * sizeof(cx<S>);
                         a.phase = ph;
-                        a.tw = G.tw[0][ax];
+                        a.tw = G.tw[L][ax];
                         a.flags = kFlagImag;
-                        batch_geom(a, in, G, 0, ax);
+                        batch_geom(a, in, G, L, ax);
                     }
                     SFD_HIP(hipMemcpyAsync(lv.d_desc, desc.data(), desc.size() * sizeof(SweepArgs),
                                            hipMemcpyHostToDevice, b->stream));
-                    const VariantInfo& vi = G.vi[0][ax];
-                    dim3 grid((unsigned)G.ntiles[0][ax], 1, (unsigned)G.count);
+                    const VariantInfo& vi = G.vi[L][ax];
+                    dim3 grid((unsigned)G.ntiles[L][ax], 1, (unsigned)G.count);
                     dim3 block(vi.ts * vi.g * vi.n2 * vi.h);
                     auto launch = [&]() {
                         return launch_batch<P>(n, ax == 0, v, grid, block, b->stream, lv.d_desc, lv.d_done,
-                                               ax == fold_axis);
+                                               ax == fold_axis || ax == 0);
                     };
                     for (int w = 0; w < 2; ++w)
                         if (!launch()) return fail(SFD_ERR_SHAPE, "batch autotune: no kernel %d for n = %d", v, n);
@@ -1302,7 +1327,7 @@ int batch_autotune(sfd_batch* b, float* times)
                         best_v = v;
                     }
                 }
-                SFD_TRY(batch_level_variant<P>(b, 0, (int)gi, ax, best_v));
+                SFD_TRY(batch_level_variant<P>(b, L, (int)gi, ax, best_v));
             }
         }
         return SFD_OK;
@@ -1346,7 +1371,7 @@ PrecOps make_ops()
     o.batch_level_setup = &batch_level_setup<P>;
     o.batch_level_variant = &batch_level_variant<P>;
     o.batch_level_bind = &batch_level_bind<P>;
-    o.batch_level_step = &batch_level_step<P>;
+    o.batch_level_steps = &batch_level_steps<P>;
     o.batch_escalate = &batch_escalate<P>;
     o.batch_phase0 = &batch_phase0<P>;
     o.batch_finish = &batch_finish<P>;

@@ -219,7 +219,7 @@ struct sfd_ctx {
 // ------------------------------------------------------------ batch solver
 struct sfd_batch {
     static constexpr int kDiffBlocks = 64;
-    static constexpr int kMaxKinds = 5;  // sweeps of one imaginary-time Strang step
+    static constexpr int kMaxKinds = 7;  // sweep descriptors of a group: a step and the chained sweeps
     static constexpr int kLevels = 2;    // 0: the batch's precision, 1: FP64 after escalation
     struct Inst {
         int ndim;
@@ -232,11 +232,13 @@ struct sfd_batch {
         double* geo[3];  // device FP64 wavenumbers [0, n) and coordinates [n, 2n) per axis
     };
     // One sweep of a step: axis, fused flags (kFlagScale / kFlagNL /
-    // kFlagNorm) and whether it is the double round trip [d N d].
+    // kFlagNorm / kFlagNormMid) and whether it is a double round trip
+    // ([d N d], or [x x] of chained steps).
     struct Kind {
         int axis, flags;
         bool twice;
     };
+    static constexpr int kFinalize = -1;  // in a sweep list: the renormalisation that ends a step
     struct Group {
         int ndim;
         int n[3];
@@ -247,7 +249,13 @@ struct sfd_batch {
         void* tw[kLevels][3] = {};
         long long ntiles[kLevels][3] = {};
         int ntx[kLevels][3] = {};
-        std::vector<Kind> kinds;  // the step of the current solve
+        // The sweeps of the current solve (kinds[k] has descriptor set k) and
+        // sequences of them: `step` is one Strang step; chained steps run
+        // `open`, then `body` once per step after the first, then `close`
+        // (chain = false: the group does not chain its steps).
+        std::vector<Kind> kinds;
+        std::vector<int> step, open, body, close;
+        bool chain = false;
     };
     // Device state of one precision level.
     struct Level {
@@ -401,12 +409,12 @@ struct PrecOps {
     int (*batch_level_variant)(sfd_batch* b, int level, int group, int axis, int idx);
     int (*batch_level_bind)(sfd_batch* b, int level, void* const* psi, const void* const* V, const double* beta,
                             const double* Omega, double cD, double dt);
-    int (*batch_level_step)(sfd_batch* b, int level, int group, hipStream_t st, bool check, double tol,
-                            bool escalate);
+    int (*batch_level_steps)(sfd_batch* b, int level, int group, hipStream_t st, int k, double tol,
+                             bool escalate);
     int (*batch_escalate)(sfd_batch* b, int group, hipStream_t st);
     int (*batch_phase0)(sfd_batch* b);
     int (*batch_finish)(sfd_batch* b);
-    int (*batch_autotune)(sfd_batch* b, float* times);
+    int (*batch_autotune)(sfd_batch* b, int level, float* times);
 };
 
 extern const PrecOps kOpsF32, kOpsF64, kOpsF32X;

@@ -939,9 +939,10 @@ __global__ __launch_bounds__(TS * G * N2 * H, W) void k_sweep(const SweepArgs a)
 // pending scale, partial sums) sits in descs[]; instances whose done flag is
 // set are skipped.  All instances of one launch share their extents, N1, N2,
 // H and the workgroup shape; grid.x*grid.y covers the tile count.  TWICE
-// runs K(tau) N K(tau) on the lines (the folded Strang step of
-// kRoundTripTwice).  Imaginary-time multipliers only (FP32: real decay
-// tables).
+// runs two round trips on the lines in one pass: K(tau) N K(tau) (the folded
+// Strang step of kRoundTripTwice) or, without kFlagNL, K(tau) K(tau) (the x
+// flow that ends one chained step and the one that starts the next).
+// Imaginary-time multipliers only (FP32: real decay tables).
 template <typename S, typename R, int N1, int N2, int TS, int G, bool TWICE = false, int H = 1, bool TL = false>
 __global__ __launch_bounds__(TS * G * N2 * H) void k_batch_sweep(const SweepArgs* __restrict__ descs,
                                                                  const int* __restrict__ done)
@@ -986,19 +987,23 @@ __global__ __launch_bounds__(TS * G * N2 * H) void k_batch_sweep(const SweepArgs
         if (a.phase_row_stride == 0) ph = tabs + NTW;
     }
     if constexpr (TWICE) {
+        // the norm partials after the second round trip (kFlagNorm) or of
+        // the state between the two (kFlagNormMid: [x x] of chained steps,
+        // whose first round trip ends a step)
+        const int mid = a.flags & kFlagNormMid;
         SweepArgs a1 = a;
-        a1.flags = a.flags & ~kFlagNorm;
+        a1.flags = (a.flags & ~(kFlagNorm | kFlagNormMid)) | (mid ? kFlagNorm : 0);
         round_trip_tile<S, R, N1, N2, TS, H * G, true, false, H>(v, vp, gm, a1, lds, twA, twB, twC, ph, c, gh, t,
                                                                  h, nacc);
         SweepArgs a2 = a;
-        a2.flags = a.flags & ~(kFlagNL | kFlagScale);
+        a2.flags = a.flags & ~(kFlagNL | kFlagScale | kFlagNormMid);
         round_trip_tile<S, R, N1, N2, TS, H * G, true, false, H>(v, vp, gm, a2, lds, twA, twB, twC, ph, c, gh, t,
                                                                  h, nacc);
     } else {
         round_trip_tile<S, R, N1, N2, TS, H * G, true, false, H>(v, vp, gm, a, lds, twA, twB, twC, ph, c, gh, t, h,
                                                                  nacc);
     }
-    if (a.flags & kFlagNorm) {
+    if (a.flags & (kFlagNorm | kFlagNormMid)) {
         double acc1[1] = {nacc};
         block_partials<R, Sh::NT, 1>(acc1, a.partial, a.ntiles, tile);
     }

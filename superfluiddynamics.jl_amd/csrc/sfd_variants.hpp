@@ -193,19 +193,16 @@ struct Dispatch<P, MODE, VL<V0, Vs...>> {
         }
         return Dispatch<P, MODE, VL<Vs...>>::launch(idx - 1, grid, block, s, a, persistent);
     }
-    // twice: K(tau) N K(tau) per line (strided variants only)
+    // twice: two round trips per line in one pass ([y N y] / [z N z] on
+    // strided variants, [x x] of chained steps on contiguous ones)
     static bool launch_batch(int idx, dim3 grid, dim3 block, hipStream_t s, const SweepArgs* descs,
                              const int* done, bool twice)
     {
         if (idx == 0) {
-            if constexpr (V0::TS == 1) {
-                if (twice) return false;
-            } else {
-                if (twice) {
-                    hipLaunchKernelGGL((k_batch_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, true, V0::H, V0::T>),
-                                       grid, block, 0, s, descs, done);
-                    return true;
-                }
+            if (twice) {
+                hipLaunchKernelGGL((k_batch_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, true, V0::H, V0::T>), grid,
+                                   block, 0, s, descs, done);
+                return true;
             }
             hipLaunchKernelGGL((k_batch_sweep<S, R, V0::N1, V0::N2, V0::TS, V0::G, false, V0::H, V0::T>), grid,
                                block, 0, s, descs, done);

@@ -153,6 +153,26 @@ def test_batch_rotation_and_3d_equal_single_field_runs(gpu, prec, graph):
         assert gpu.norm(f) == pytest.approx(1.0, rel=1e-6)
 
 
+@pytest.mark.parametrize("check_every", [1, 2, 3, 4, 37])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_batch_chunk_lengths_equal_single_field_runs(gpu, prec, check_every):
+    """Non-rotating groups chain the steps of a chunk before its check step
+    (x, [y N y], [x x], ..., x in 2D; x, y, [z N z], y, [x x], ... in 3D);
+    chunks of 1 and 2 steps do not chain, 3 chains two steps, 4 three.  Every
+    chunk length ends where the single-field runs end, rotating groups
+    beside them included."""
+    fields, params, inits = _make_mixed(gpu, prec, seed=17)
+    steps, dpsi = gpu.BatchImagSolver(fields, params, 4e-3).run(max_steps=37, tol=0.0, check_every=check_every)
+    assert list(steps) == [37] * len(fields)
+    assert np.all(dpsi > 0)
+    for f, (psi0, V, bounds, beta, Om) in zip(fields, inits):
+        ref = gpu_run(gpu, psi0, bounds, V=V, beta=beta, Omega=Om, dt=4e-3, nsteps=37, order=2, imag=True,
+                      prec=prec)
+        err = rel_l2(f.to_numpy(), ref)
+        assert err <= (1e-13 if prec == "f64" else 1e-6), (f.g.n, Om, check_every, err)
+        assert gpu.norm(f) == pytest.approx(1.0, rel=1e-6)
+
+
 def test_batch_rotation_and_3d_match_oracle(gpu, orc):
     fields, params, inits = _make_mixed(gpu, "f64", seed=13)
     gpu.BatchImagSolver(fields, params, 5e-3).run(max_steps=20, tol=0.0, check_every=7)
