@@ -47,11 +47,11 @@ def bytes_per_site_update(prec: str, ndim: int = 3, with_v: bool = False, passes
 def passes_per_step(ndim: int, steps: int, chained: bool = True) -> float:
     """Sweeps per Strang step of one call of `steps` steps (Omega = 0, real
     time): x, y, [z N z], x, y per step (2D - 1) when each step runs alone;
-    x, y, [z N z], y, [x x], y, [z N z], ..., y, x when the call chains them
-    ((2D - 2) per step plus the closing x sweep)."""
+    y(τ), z(τ), x(τ) + N, y(Δt), z(Δt), x(Δt) + N, ..., y(τ), z(τ), x(τ) when
+    the call chains them (D per step plus the D closing sweeps)."""
     if not chained or steps < 2:
         return float(2 * ndim - 1)
-    return ((2 * ndim - 2) * steps + 1) / steps
+    return ndim * (steps + 1) / steps
 
 
 def load_traffic(slot: str, n: int = 256):
@@ -131,14 +131,14 @@ def gpu_leg(sfd, torch, psi0, mp, prec, steps, warmup, reps, launches, settle_ms
         if cnt[k]:
             sweeps[name] = {"launches_per_step": cnt[k] / steps, "avg_ms": tot[k] / cnt[k],
                             "ms_per_step": tot[k] / steps}
-    # The one-axis round trips of the step (x, y: the same kernels, variants
-    # and flags as in the step) launched back to back on the context's
-    # stream: HIP events around `launches` launches, median of 3 batches.
-    # The in-step events above add a few microseconds to every launch; these
-    # do not.
+    # The one-axis round trips of the chained step that carry no nonlinear
+    # phase (y, z: the same kernels, variants and flags as in the step)
+    # launched back to back on the context's stream: HIP events around
+    # `launches` launches, median of 3 batches.  The in-step events above add
+    # a few microseconds to every launch; these do not.
     back_to_back = {}
     psi = ctypes.c_void_p(f.ϕ.data_ptr())
-    for ax, name in ((0, "x round trip"), (1, "y round trip")):
+    for ax, name in ((1, "y round trip"), (2, "z round trip")):
         reps = []
         for _ in range(3):
             k0 = torch.cuda.Event(enable_timing=True)
@@ -203,6 +203,21 @@ def kernel_name(variants, slot):
     return f"{'k_sweep_pipe' if pipe else 'k_sweep'}<float, float, {n1}, {n2}, {ts}, {g}>"
 
 
+def dominant_kernel(variants, sweeps):
+    """(kernel, slots): the sweep kernel with the largest share of the step's
+    time, summed over the timer slots it runs in (`sweeps`: slot name ->
+    {"ms_per_step", ...}), and those slots in slot order."""
+    share, slots = {}, {}
+    for k, name in enumerate(SWEEP_SLOTS):
+        if name == "other sweeps" or name not in sweeps:
+            continue
+        kn = kernel_name(variants, k)
+        share[kn] = share.get(kn, 0.0) + sweeps[name]["ms_per_step"]
+        slots.setdefault(kn, []).append(name)
+    kn = max(share, key=share.get)
+    return kn, slots[kn]
+
+
 def cpu_leg(psi0, mp, cpu_steps):
     """The FP64 CPU oracle (C + OpenMP restatement of NumModelSplit2) on the
     same 256³ workload: one untimed step, then cpu_steps timed steps
@@ -243,9 +258,11 @@ def solve_driver(n=256, iters=20, prec="f32"):
 def traffic_driver(n=256, launches=5):
     """Launches for the rocprofv3 --pmc runs: `launches` round trips of each
     axis of the 256³ FP32 workload under the autotuned variants (x: the
-    contiguous kernel, then y and z: the strided kernel), `launches` Split2
-    steps (whose [z N z] sweep is the double round trip), then `launches`
-    in-place passes of a plain elementwise
+    contiguous kernel, then y and z: the strided kernel), one call of
+    `launches` chained Split2 steps (y, z, x + N), `launches` one-step calls
+    (x, y, [z N z], x, y: the innermost double round trip), one call of
+    `launches` chained imaginary-time steps (whose [x x] is the x double
+    round trip), then `launches` in-place passes of a plain elementwise
     kernel over the same 128 MiB (torch mul_), the calibration whose
     algorithmic traffic is exactly one read and one write of the field."""
     import ctypes
@@ -269,7 +286,16 @@ def traffic_driver(n=256, launches=5):
     rc = lib.sfd_split_step(ctx.h, psi, None, mp["coeffD"], mp["beta"], 0.0, mp["dt"], launches, 2, 0)
     assert rc == 0, lib.sfd_last_error()
     torch.cuda.synchronize()
-    print(f"[traffic] {launches} Split2 steps", flush=True)
+    print(f"[traffic] {launches} chained Split2 steps", flush=True)
+    for _ in range(launches):
+        rc = lib.sfd_split_step(ctx.h, psi, None, mp["coeffD"], mp["beta"], 0.0, mp["dt"], 1, 2, 0)
+        assert rc == 0, lib.sfd_last_error()
+    torch.cuda.synchronize()
+    print(f"[traffic] {launches} one-step Split2 calls", flush=True)
+    rc = lib.sfd_split_step(ctx.h, psi, None, mp["coeffD"], mp["beta"], 0.0, mp["dt"], launches, 2, 1)
+    assert rc == 0, lib.sfd_last_error()
+    torch.cuda.synchronize()
+    print(f"[traffic] {launches} chained imaginary-time Split2 steps", flush=True)
     flat = torch.view_as_real(f.ϕ)
     for _ in range(launches):
         flat.mul_(1.0)
@@ -404,17 +430,22 @@ def main(argv=None):
                          variants=r["variants"])
 
     # Dominant kernel: the sweep kernel with the largest share of the FP32
-    # step's time (in-step timer), one read and one write of the complex FP32
-    # field per launch (V = 0: no potential bytes).  Its average duration is
-    # the back-to-back figure when it is a one-axis round trip (x or y), else
-    # the in-step one.
+    # step's time (in-step timer, the slots of one kernel summed: the y and z
+    # round trips of a chained step are one kernel when they share a
+    # variant), one read and one write of the complex FP32 field per launch
+    # (V = 0: no potential bytes).  Its average duration is the back-to-back
+    # figure of its slots where there is one (y, z), else the in-step one,
+    # weighted by launches per step.
     k32 = res["f32"]["sweeps"]
-    dom = max((k for k in k32 if k != "other sweeps"), key=lambda k: k32[k]["ms_per_step"])
-    dom_ms = res["f32"]["back_to_back"].get(dom, k32[dom]["avg_ms"])
+    dom_kernel, dom_slots = dominant_kernel(res["f32"]["variants"], k32)
+    b2b = res["f32"]["back_to_back"]
+    nl_ = sum(k32[sl]["launches_per_step"] for sl in dom_slots)
+    dom_ms = sum(b2b.get(sl, k32[sl]["avg_ms"]) * k32[sl]["launches_per_step"] for sl in dom_slots) / nl_
+    dom_ms_in_step = sum(k32[sl]["ms_per_step"] for sl in dom_slots) / nl_
+    dom_share = sum(k32[sl]["ms_per_step"] for sl in dom_slots)
     launch_bytes = 2 * 8 * sites
     achieved = launch_bytes / (dom_ms * 1e-3)
-    traffic = load_traffic(dom, n)
-    dom_kernel = kernel_name(res["f32"]["variants"], SWEEP_SLOTS.index(dom))
+    traffic = next((t for t in (load_traffic(sl, n) for sl in dom_slots) if t is not None), None)
     copy_ms = res["f32"]["copy_ms"]
 
     # Accuracy at the largest stable step: at dt = 5e-4 the unfiltered split is
@@ -466,7 +497,8 @@ def main(argv=None):
         "f32_over_f64_speedup": out["f64"]["ms_per_step"] / out["f32"]["ms_per_step"],
         "f32x_over_f64_speedup": out["f64"]["ms_per_step"] / out["f32x"]["ms_per_step"],
         "f32_step_hbm_frac": out["f32"]["step_hbm_frac"],
-        "f32_step_hbm_frac_note": f"a call of {args.steps} chained steps (x, y, [z N z], y, [x x], y, ..., y, x) "
+        "f32_step_hbm_frac_note": f"a call of {args.steps} chained steps (y, z, x + N per step: the half-step "
+                                  "flows between two nonlinear phases merged into one round trip per axis) "
                                   f"moves {passes_per_step(3, args.steps):.3f} passes x 16 B = "
                                   f"{out['f32']['bytes_per_site_update']:.2f} B per site-update; priced at the "
                                   "6-pass 96 B of the reference's structure it would read "
@@ -492,10 +524,10 @@ def main(argv=None):
                        **{p_: res[p_]["solve"] for p_ in ("f32", "f32x", "f64")}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": f"{dom_kernel}: {dom}, FP32, {n}³ (the step's largest share of time, "
-                               f"{k32[dom]['ms_per_step'] * 1e3:.1f} us of {out['f32']['ms_per_step'] * 1e3:.1f} "
+                     "kernel": f"{dom_kernel}: {' and '.join(dom_slots)}s, FP32, {n}³ (the step's largest "
+                               f"share of time, {dom_share * 1e3:.1f} us of {out['f32']['ms_per_step'] * 1e3:.1f} "
                                f"us per step)",
-                     "avg_launch_ms": dom_ms, "avg_launch_ms_in_step": k32[dom]["avg_ms"],
+                     "avg_launch_ms": dom_ms, "avg_launch_ms_in_step": dom_ms_in_step,
                      "bytes_per_launch": launch_bytes,
                      "timing": "avg_launch_ms: HIP events around back-to-back launches on the context's stream; "
                                "avg_launch_ms_in_step: events around each launch inside the timed steps "

@@ -72,12 +72,17 @@ int sfd_ctx_set_stream(sfd_ctx* ctx, void* stream);
  * to norm 1 (normalize!, Fields.jl:196-200) after each step.
  * Sweep order of a Strang step: the innermost axis's two half-step flows
  * commute with the other axes' and meet around the nonlinear phase in one
- * pass, x, y, [z N z], x, y (2D without rotation: x, [y N y], x).  In real
- * time with Omega = 0 every flow commutes, and a call of several steps runs
- * each step as x, y, [z N z], y, x, so that the x flows of consecutive steps
- * share one pass, [x x]: 2D - 2 passes over the field per step.  Each
- * half-step flow keeps its own round trip; only the order of commuting flows
- * changes, so the results equal the reference order's up to rounding. */
+ * pass, x, y, [z N z], x, y (2D without rotation: x, [y N y], x).  With
+ * Omega = 0 every flow commutes, and a call of several steps shares flows
+ * between steps: in real time the half-step flows that meet between two
+ * nonlinear phases run as one flow of dt per axis, y(dt/2), z(dt/2),
+ * x(dt/2) + N, y(dt), z(dt), x(dt) + N, ..., y(dt/2), z(dt/2), x(dt/2),
+ * with the nonlinear phase fused into the x round trip: D passes over the
+ * field per step.  In imaginary time (the norm of the state between
+ * steps is needed) each step runs as x, y, [z N z], y, x and the x flows of
+ * consecutive steps share one pass, [x x]: 2D - 2 passes per step.  Only the
+ * grouping of commuting linear flows changes, so the results equal the
+ * reference order's up to rounding. */
 int sfd_split_step(sfd_ctx* ctx, void* psi, const void* V, double coeffD, double beta,
                    double Omega, double dt, int64_t nsteps, int order, int mode);
 

@@ -1088,18 +1088,25 @@ int sfd_split_step_noalloc(sfd_ctx* ctx, void* psi, const void* V, double coeffD
     unsigned char* base = static_cast<unsigned char*>(scratch_ptr(ctx->device));
     if (!base) return fail(SFD_ERR_HIP, "sfd_split_step_noalloc: no scratch array on device %d", ctx->device);
     const PrecOps& ops = ops_for(ctx->prec);
-    // layout: [scale and sums: 256 B][norm partials][phase table of each axis]
+    // layout: [scale and sums: 256 B][norm partials][phase table of each
+    // axis for tau][for dt, when the call chains real-time steps]
     size_t need = 256;
     long long nparts = 0;
     for (int ax = 0; ax < ctx->ndim; ++ax) nparts = std::max(nparts, sweep_nblocks(ctx, ax));
     const size_t part_off = need;
     if (mode == SFD_IMAG_TIME) need += align256((size_t)nparts * sizeof(double));
+    const bool whole = mode == SFD_REAL_TIME && order == 2 && Omega == 0.0 && nsteps >= 2 && ctx->ndim >= 2;
     Workspace ws{};
-    size_t tab_off[3] = {0, 0, 0};
+    size_t tab_off[3] = {0, 0, 0}, dt_off[3] = {0, 0, 0};
     for (int ax = 0; ax < ctx->ndim; ++ax) {
         ws.rows[ax] = phase_rows(ctx, ax, Omega);
+        const size_t tb = align256((size_t)ws.rows[ax] * ctx->n[ax] * ops.table_entry_bytes);
         tab_off[ax] = need;
-        need += align256((size_t)ws.rows[ax] * ctx->n[ax] * ops.table_entry_bytes);
+        need += tb;
+        if (whole) {
+            dt_off[ax] = need;
+            need += tb;
+        }
     }
     if (need > kScratchBytes)
         return fail(SFD_ERR_ARG,
@@ -1110,9 +1117,15 @@ int sfd_split_step_noalloc(sfd_ctx* ctx, void* psi, const void* V, double coeffD
     ws.partial = reinterpret_cast<double*>(base + part_off);
     const double tau = order == 2 ? 0.5 * dt : dt;
     const int imag = mode == SFD_IMAG_TIME;
+    ws.tau = tau;
     for (int ax = 0; ax < ctx->ndim; ++ax) {
         ws.phase[ax] = base + tab_off[ax];
         SFD_TRY(ops.build_phase_into(ctx, ctx->stream, ws.phase[ax], ax, tau, coeffD, Omega, imag, ws.rows[ax]));
+        if (whole) {
+            ws.phase_dt[ax] = base + dt_off[ax];
+            SFD_TRY(ops.build_phase_into(ctx, ctx->stream, ws.phase_dt[ax], ax, dt, coeffD, Omega, imag,
+                                         ws.rows[ax]));
+        }
     }
     return ops.split(ctx, psi, V, coeffD, beta, Omega, dt, nsteps, order, mode, nullptr, kFoldChain, &ws);
 }

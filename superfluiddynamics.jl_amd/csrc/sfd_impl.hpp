@@ -224,11 +224,15 @@ int build_phase(sfd_ctx* c, int axis, double tau, double cD, double Om, int imag
 template <typename P, int MODE>
 int launch_sweep(sfd_ctx* c, int axis, SweepArgs a)
 {
-    // The pipelined kernel of a pipelined variant takes round trips without a
-    // nonlinear epilogue, pending scale or energy sums and with one phase
-    // row; other sweeps run k_sweep.
+    // The pipelined kernel of a pipelined variant takes round trips with one
+    // phase row, without a pending scale or energy sums, and with a
+    // nonlinear epilogue only at V = 0 that stores no pre-phase state: its
+    // loop issues no global memory operation but the tiles' loads and stores
+    // (it waits for the prefetched tile by counting them).  Other sweeps run
+    // k_sweep.
     const bool persistent = c->vi[axis].pipe && (MODE == kRoundTrip || MODE == kRoundTripImag) &&
-                            !(a.flags & (kFlagNL | kFlagScale)) && a.phase_row_stride == 0;
+                            !(a.flags & kFlagScale) && !((a.flags & kFlagNL) && (a.V || a.save)) &&
+                            a.phase_row_stride == 0;
     const bool blu = c->blu[axis];
     Launch L = sweep_geometry(c, axis, a, persistent && !blu);
     a.tw = c->tw[axis];
@@ -296,7 +300,8 @@ int set_variant(sfd_ctx* c, int axis, int idx)
 // (kRoundTripEnergy; kFlagPointwise in `flags` adds the pointwise sums).
 // With `twice`, the sweep is K(tau) N K(tau) on this axis (kRoundTripTwice):
 // the nonlinear phase of `flags` between two round trips.  With `ws`, the
-// phase table, scale and partials come from it (its tables are for tau).
+// phase table, scale and partials come from it (its tables are for its tau
+// and, where built, for dt).
 template <typename P>
 int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, double cD, double Om, int imag,
                int flags, double beta, double dt, void* save = nullptr, double* epartial = nullptr,
@@ -308,7 +313,8 @@ int round_trip(sfd_ctx* c, void* psi, const void* V, int axis, double tau, doubl
     void* ph = nullptr;
     int rows = 1;
     if (ws) {
-        ph = ws->phase[axis];
+        ph = tau == ws->tau ? ws->phase[axis] : ws->phase_dt[axis];
+        if (!ph) return fail(SFD_ERR_ARG, "workspace: no phase table for tau = %g on axis %d", tau, axis);
         rows = ws->rows[axis];
     } else {
         SFD_TRY(build_phase<P>(c, axis, tau, cD, Om, imag, &ph, &rows));
@@ -391,14 +397,18 @@ template <typename P>
 // per step instead of 2D).  `fold = kFoldNone` keeps the x, y, z, N, x, y, z
 // order of splitting.jl:149-154.
 //
-// With `fold = kFoldChain`, real time, Omega = 0 and several steps, every
-// flow commutes with every other, so each step may run its second half in
-// reverse, x, y, [z N z], y, x; the x flow ending step s and the one
-// starting step s + 1 then meet in one double round trip [x x] (two round
-// trips of tau each, in one read and one write of the field): x, y,
+// With `fold = kFoldChain`, Omega = 0 and several steps, every flow commutes
+// with every other.  In real time the half-step flows that meet between two
+// nonlinear phases are one flow of dt per axis, K(tau) N K(dt) N ... N
+// K(tau), with the nonlinear phase in the epilogue of the x round trip: D
+// sweeps per step, each a single round trip.  Where the state
+// between two steps is needed (the norm in imaginary time, the energy sums
+// of solve!), each step runs its second half in reverse, x, y, [z N z], y,
+// x, and the x flow ending step s and the one starting step s + 1 meet in
+// one double round trip [x x] (two round trips of tau each, in one read and
+// one write of the field, the state between them in registers): x, y,
 // [z N z], y, [x x], y, [z N z], ..., y, x, i.e. 2D - 2 sweeps per step
-// instead of 2D - 1 (2D: x, [y N y], [x x], [y N y], ..., x).  Every
-// half-step flow is still applied by its own round trip.
+// instead of 2D - 1 (2D: x, [y N y], [x x], [y N y], ..., x).
 int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, double Om, double dt,
                int64_t nsteps, int order, int mode, double* ehist = nullptr, int fold = kFoldInner,
                const Workspace* ws = nullptr)
@@ -431,7 +441,24 @@ int split_impl(sfd_ctx* c, void* psi, const void* V, double cD, double beta, dou
     }
     bool pending = false;
     if (!(order == 2 && D >= 2 && (Om == 0.0 || D == 3))) fold = kFoldNone;
-    if (fold == kFoldChain) {
+    if (fold == kFoldChain && Om == 0.0 && nsteps >= 2 && !imag && !ehist) {
+        // Real time without energy sums: the kinetic flow between two
+        // nonlinear phases is K(dt) = x(dt) y(dt) z(dt), one round trip per
+        // axis in any order, and the phase is the epilogue of the x round
+        // trip (the contiguous sweep, which hides its cost best): y(tau),
+        // z(tau), x(tau) + N, then y(dt), z(dt), x(dt) + N per further step,
+        // then y(tau), z(tau), x(tau).
+        for (int64_t s = 0; s <= nsteps; ++s) {
+            const double t = (s == 0 || s == nsteps) ? tau : dt;
+            for (int k = 1; k <= D; ++k) {
+                const int ax = k % D;  // y, z, then x with the nonlinear phase
+                SFD_TRY(round_trip<P>(c, psi, V, ax, t, cD, Om, imag, (s < nsteps && ax == 0) ? nlflags : 0,
+                                      beta, dt, nullptr, nullptr, false, ws));
+            }
+        }
+        nsteps = 0;
+    }
+    if (fold == kFoldChain && nsteps) {
         if (Om == 0.0 && nsteps >= 2 && !c->blu[0] && !(imag && ehist)) {
             // With `ehist`, the energy of the state before step s comes out of
             // the sweeps that first transform it: x from the opening x round
@@ -860,12 +887,14 @@ int autotune_impl(sfd_ctx* c, void* scratch, float* times_out)
         const int nv = axis_variants<P>(c, ax);
         int best = 0;
         float best_ms = 1e30f;
-        // The innermost axis of a Strang step runs the double round trip
-        // [d N d] (z in 3D; y in 2D without rotation) besides its single
-        // round trips (Split1, rotating 2D steps, transforms), and the x axis
-        // the double round trip [x x] of chained steps: their variants are
-        // chosen by the sum of both times.
+        // Chained real-time steps run single round trips, x's with the
+        // nonlinear phase in its epilogue.  The innermost axis of a
+        // one-step call, of imaginary-time and of energy-loop steps also runs
+        // the double round trip [d N d] (z in 3D; y in 2D without rotation),
+        // and the x axis of their chained steps the double round trip [x x]:
+        // their variants are chosen by the sum of both times.
         const bool folds = c->ndim >= 2 && (ax == c->ndim - 1 || (ax == 0 && !c->blu[0]));
+        const int single_flags = (c->ndim >= 2 && ax == 0) ? kFlagNL : 0;
         for (int v = 0; rc == SFD_OK && v < nv; ++v) {
             if ((rc = set_variant<P>(c, ax, v)) != SFD_OK) break;
             float score = 0.f;
@@ -873,7 +902,7 @@ int autotune_impl(sfd_ctx* c, void* scratch, float* times_out)
                 auto launch = [&]() {
                     return twice ? round_trip<P>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, ax == 0 ? 0 : kFlagNL, 0.0,
                                                  0.0, nullptr, nullptr, true)
-                                 : round_trip<P>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, 0, 0.0, 0.0);
+                                 : round_trip<P>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, single_flags, 0.0, 0.0);
                 };
                 float vbest = 1e30f;
                 for (int w = 0; rc == SFD_OK && w < 2; ++w) rc = launch();

@@ -146,19 +146,24 @@ struct Launch {
 // partials of the renormalising sweep, and the phase table of each axis for
 // the call's single tau.
 struct Workspace {
-    double* scal;     // [0] pending scale, [1] sum |psi|^2
-    double* partial;  // per-workgroup sums of the renormalising sweep
-    void* phase[3];
+    double* scal;       // [0] pending scale, [1] sum |psi|^2
+    double* partial;    // per-workgroup sums of the renormalising sweep
+    void* phase[3];     // phase tables of the half step tau
+    void* phase_dt[3];  // phase tables of the whole step dt (chained real-time steps), or null
     int rows[3];
+    double tau;
 };
 
 // Sweep orders of the split step (split_impl):
 //   kFoldNone   x, y, z, N, x, y, z: the order of splitting.jl:149-154;
 //   kFoldInner  x, y, [z N z], x, y: the innermost axis's two flows meet
 //               around the nonlinear phase in one double round trip;
-//   kFoldChain  also x, y, [z N z], y, x per step, so that the last x flow
-//               of a step and the first of the next meet in one double
-//               round trip [x x] (real time, Omega = 0, several steps).
+//   kFoldChain  several steps at Omega = 0 share their flows: in real
+//               time y(tau), z(tau), x(tau) + N, y(dt), z(dt), x(dt) + N,
+//               ..., y(tau), z(tau), x(tau); where the state between two
+//               steps is needed, x, y, [z N z], y, x per step, so that the
+//               last x flow of a step and the first of the next meet in one
+//               double round trip [x x].
 enum FoldLevel : int { kFoldNone = 0, kFoldInner = 1, kFoldChain = 2 };
 
 }  // namespace sfd

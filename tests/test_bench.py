@@ -23,10 +23,10 @@ def test_bytes_per_site_update(prec, ndim, with_v, passes, want):
 
 @pytest.mark.parametrize("ndim, steps, chained, want", [
     (3, 1, True, 5.0),         # one step alone: x, y, [z N z], y, x
-    (3, 2, True, 4.5),         # x, y, [z N z], y, [x x], y, [z N z], y, x: 9 sweeps
-    (3, 200, True, 4.005),
+    (3, 2, True, 4.5),         # y, z, x + N, y, z, x + N, y, z, x: 9 sweeps
+    (3, 200, True, 3.015),
     (3, 200, False, 5.0),
-    (2, 4, True, 2.25),        # x, [y N y], [x x], [y N y], [x x], ..., x: 9 sweeps
+    (2, 4, True, 2.5),         # y, x + N, y, x + N, ..., y, x: 10 sweeps
 ])
 def test_passes_per_step(ndim, steps, chained, want):
     assert bench.passes_per_step(ndim, steps, chained) == pytest.approx(want)
@@ -60,6 +60,18 @@ def test_kernel_names_of_the_sweep_slots():
     assert bench.kernel_name(v, 0) == "k_sweep_pipe<float, float, 16, 16, 1, 8>"
     assert bench.kernel_name(v, 2) == "k_sweep<float, float, 16, 16, 16, 1>"
     assert bench.kernel_name(v, 3) == "k_sweep<float, float, 16, 16, 16, 1, 7>"
+
+
+def test_dominant_kernel_sums_the_slots_of_one_kernel():
+    """The y and z round trips of a chained step share a kernel when they
+    share a variant: their times add up against the x round trip's."""
+    v = [(16, 16, 1, 8, 1, 1, 1), (16, 16, 16, 1, 1, 1, 1), (16, 16, 16, 1, 1, 1, 1)]
+    sw = {"x round trip": {"ms_per_step": 0.050}, "y round trip": {"ms_per_step": 0.045},
+          "z round trip": {"ms_per_step": 0.046}, "other sweeps": {"ms_per_step": 0.2}}
+    assert bench.dominant_kernel(v, sw) == ("k_sweep_pipe<float, float, 16, 16, 16, 1>",
+                                            ["y round trip", "z round trip"])
+    v2 = [v[0], v[1], (16, 16, 8, 1, 1, 1, 1)]
+    assert bench.dominant_kernel(v2, sw) == ("k_sweep_pipe<float, float, 16, 16, 1, 8>", ["x round trip"])
 
 
 def test_more_than_one_gpu_is_refused(capsys):

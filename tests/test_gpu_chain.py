@@ -1,8 +1,13 @@
-"""GPU checks of chained Strang steps: a call of several real-time steps with
-Omega = 0 runs x, y, [z N z], y, [x x], y, [z N z], ..., y, x, where [x x] is
-one pass holding the x round trip that ends a step and the one that starts
-the next (splitting.jl:149-154 applies x, y, z, N, x, y, z per step; every
-flow commutes with every other at Omega = 0, so only rounding differs).
+"""GPU checks of chained Strang steps (splitting.jl:149-154 applies x, y, z,
+N, x, y, z per step; every flow commutes with every other at Omega = 0, so
+only rounding differs):
+- real time: a call of several steps runs y(τ), z(τ), x(τ) + N, then y(Δt),
+  z(Δt), x(Δt) + N per further step, then y(τ), z(τ), x(τ): the half-step
+  flows that meet between two nonlinear phases are one round trip per axis,
+  and the phase is the epilogue of the x round trip;
+- imaginary time (the norm of the state between two steps is needed): x, y,
+  [z N z], y, [x x], y, [z N z], ..., y, x, where [x x] is one pass holding
+  the x round trip that ends a step and the one that starts the next.
 
 Bars: FP64 relative L2 <= 1e-12 against the oracle and <= 1e-13 against the
 same steps taken one call at a time; FP32 <= 2e-5 against the oracle.
@@ -49,6 +54,8 @@ CASES = [
     ((32, 16, 64), ((-4, 4), (-3, 3), (-5, 5)), True),
     ((64, 64, 64), ((0, 2 * np.pi),) * 3, False),
     ((512, 8, 4), ((-8, 8), (-1, 1), (-1, 1)), True),
+    ((48, 16, 32), ((-4, 4), (-3, 3), (-5, 5)), True),   # Bluestein x
+    ((24, 40), ((-4, 4), (-5, 5)), False),                # Bluestein x and y
 ]
 
 
@@ -65,13 +72,10 @@ def test_chained_steps_match_oracle_and_single_steps(gpu, orc, case, prec):
     counts = _launch_counts(gpu, f, lambda: m.steps(K))
     chained = f.to_numpy().astype(np.complex128)
     D = len(shape)
-    # one [x x] per step boundary, one [d N d] per step, the innermost-but-one
-    # axes twice per step, and the opening and closing x round trips
-    assert counts[5] == K - 1, counts
-    assert counts[3] == K, counts
-    assert counts[0] == 2, counts
-    if D == 3:
-        assert counts[1] == 2 * K, counts
+    # one round trip per axis and step, plus the closing ones; no double
+    # round trips
+    assert counts[3] == 0 and counts[5] == 0, counts
+    assert counts[:D] == [K + 1] * D, counts
     f.set(psi0)
     for _ in range(K):
         assert gpu.timeStep(m) == 2
@@ -88,17 +92,17 @@ def test_chained_steps_match_oracle_and_single_steps(gpu, orc, case, prec):
         assert e_ref < 2e-5
 
 
-@pytest.mark.parametrize("kind", ["rotating", "bluestein-x", "one-step"])
+@pytest.mark.parametrize("kind", ["rotating", "bluestein-x-imaginary", "one-step"])
 def test_chain_only_where_every_flow_commutes(gpu, orc, kind):
-    """Rotation (x and y flows do not commute), a Bluestein x axis and
-    single-step calls keep the x, y, [z N z], x, y order: no [x x] pass;
-    results still match the oracle."""
-    shape_x, bounds = ((48, 16, 32), ((-4, 4), (-3, 3), (-5, 5))) if kind == "bluestein-x" else \
+    """Rotation (x and y flows do not commute), single-step calls and
+    imaginary-time steps on a Bluestein x axis (no [x x] on Bluestein lines)
+    keep the x, y, [z N z], x, y order; results still match the oracle."""
+    shape_x, bounds = ((48, 16, 32), ((-4, 4), (-3, 3), (-5, 5))) if kind.startswith("bluestein") else \
         ((32, 16, 64), ((-4, 4), (-3, 3), (-5, 5)))
     psi0 = smooth_state(shape_x, bounds, seed=3)
     V = random_potential(shape_x, bounds)
     Om = 0.4 if kind == "rotating" else 0.0
-    imag = kind == "imaginary"
+    imag = kind.endswith("imaginary")
     K = 1 if kind == "one-step" else 6
     f, m = _model(gpu, psi0, bounds, V, "f64", Omega=Om, imag=imag)
     counts = _launch_counts(gpu, f, lambda: m.steps(K))
@@ -117,16 +121,18 @@ def test_chain_only_where_every_flow_commutes(gpu, orc, kind):
     assert err < 1e-12
 
 
-def test_every_x_variant_runs_the_double_round_trip(gpu, orc):
-    """Each compiled x-axis variant (the [x x] pass takes the x variant's
-    shape) gives the oracle's result over a chained call, 2D and 3D."""
+@pytest.mark.parametrize("imag", [False, True], ids=["real", "imaginary"])
+def test_every_x_variant_runs_the_chained_steps(gpu, orc, imag):
+    """Each compiled x-axis variant gives the oracle's result over a chained
+    call, 2D and 3D: the merged x(Δt) round trips in real time, the [x x]
+    pass (which takes the x variant's shape) in imaginary time."""
     for shape_x, bounds in (((256, 16), ((-8, 8), (-2, 2))), ((128, 8, 16), ((-6, 6), (-2, 2), (-3, 3)))):
         psi0 = smooth_state(shape_x, bounds, seed=11)
         V = random_potential(shape_x, bounds)
         ref = orc.split_step(psi0.copy(), bounds, V=V, coeffD=-0.5, beta=3.0, Omega=0.0, dt=2e-3, nsteps=5,
-                             order=2)
+                             order=2, imag=imag)
         for prec in ("f64", "f32"):
-            f, m = _model(gpu, psi0, bounds, V, prec)
+            f, m = _model(gpu, psi0, bounds, V, prec, imag=imag)
             ctx = f.ctx()
             for v in range(ctx.variant(0)[1]):
                 ctx.set_variant(0, v)
