@@ -153,7 +153,9 @@ int sfd_fft_axis(sfd_ctx* ctx, const void* in, void* out, int axis, int directio
 /* Autotuner (no counterpart in the reference, whose FFT plans come from
  * FFTW/CUFFT planning, Plans.jl:847-848,991-993): for each axis, time every
  * compiled launch variant of the round-trip sweep (line split N = n1*n2,
- * workgroup shape) on a scratch field and keep the fastest for this context.
+ * workgroup shape) on a scratch field and keep the fastest for this context;
+ * on the x axis and the innermost axis the double round trips of a step
+ * ([x x], [d N d]) are timed too and get their own fastest variant.
  * scratch: a device buffer of nsites complex values (overwritten with
  * unit-modulus pseudo-random data, since sweep timings depend on the data),
  * or NULL to allocate one temporarily.  times_ms (optional, 24 floats)
@@ -161,14 +163,14 @@ int sfd_fft_axis(sfd_ctx* ctx, const void* in, void* out, int axis, int directio
  * of variant v of axis a at [a*8 + v] (-1 where none).  Synchronises. */
 int sfd_autotune(sfd_ctx* ctx, void* scratch, float* times_ms);
 
-/* Current launch variant of an axis: its index, the number of variants
- * compiled for this line length and precision, and (optional) its shape
- * {n1, n2, ts, g, pipelined, minimum waves per SIMD, halves}: a line of
- * halves*n1*n2 points, halves = 2 splitting it into interleaved halves
- * joined by a radix-2 stage. */
+/* Current launch variant of an axis's single round trips: its index, the
+ * number of variants compiled for this line length and precision, and
+ * (optional) its shape {n1, n2, ts, g, pipelined, minimum waves per SIMD,
+ * halves}: a line of halves*n1*n2 points, halves = 2 splitting it into
+ * interleaved halves joined by a radix-2 stage. */
 int sfd_ctx_get_variant(sfd_ctx* ctx, int axis, int* index, int* count, int* shape7);
-/* Force a launch variant (index < count); results are identical up to
- * floating-point rounding. */
+/* Force a launch variant (index < count) on an axis's single and double
+ * round trips; results are identical up to floating-point rounding. */
 int sfd_ctx_set_variant(sfd_ctx* ctx, int axis, int index);
 
 /* Sweep timer (no counterpart in the reference, whose benchmark scripts time

@@ -176,11 +176,12 @@ int ensure_buf(sfd_ctx* c, double** buf, long long* cap, long long need)
 
 int ensure_partial(sfd_ctx* c, long long need) { return ensure_buf(c, &c->partial, &c->partial_cap, need); }
 
-// Launch geometry of one sweep under the axis's current variant: returns
-// grid/block and fills the line fields of the argument block.
-Launch sweep_geometry(const sfd_ctx* c, int axis, SweepArgs& a, bool persistent)
+// Launch geometry of one sweep under the axis's current variant (of its
+// double round trips with `twice`): returns grid/block and fills the line
+// fields of the argument block.
+Launch sweep_geometry(const sfd_ctx* c, int axis, SweepArgs& a, bool persistent, bool twice)
 {
-    const VariantInfo& v = c->vi[axis];
+    const VariantInfo& v = twice ? c->vi2[axis] : c->vi[axis];
     const int NL = v.ts * v.g;
     Launch L;
     const long long nx = c->n[0], ny = c->n[1], nz = c->n[2];
@@ -213,7 +214,7 @@ Launch sweep_geometry(const sfd_ctx* c, int axis, SweepArgs& a, bool persistent)
 long long sweep_nblocks(const sfd_ctx* c, int axis)
 {
     SweepArgs a{};
-    return sweep_geometry(c, axis, a, false).nblocks;
+    return std::max(sweep_geometry(c, axis, a, false).nblocks, sweep_geometry(c, axis, a, false, true).nblocks);
 }
 
 int check_ctx(sfd_ctx* c)
@@ -271,6 +272,7 @@ void release(sfd_ctx* c)
     c->tev.clear();
     for (int ax = 0; ax < 3; ++ax) {
         if (c->tw[ax]) (void)hipFree(c->tw[ax]);
+        if (c->tw2[ax]) (void)hipFree(c->tw2[ax]);
         if (c->xi[ax]) (void)hipFree(c->xi[ax]);
         if (c->coord[ax]) (void)hipFree(c->coord[ax]);
     }
@@ -1028,7 +1030,7 @@ int sfd_ctx_set_variant(sfd_ctx* ctx, int axis, int index)
 {
     SFD_TRY(check_ctx(ctx));
     if (axis < 0 || axis >= ctx->ndim) return fail(SFD_ERR_ARG, "axis %d out of range", axis);
-    return ops_for(ctx->prec).set_variant(ctx, axis, index);
+    return ops_for(ctx->prec).set_variant(ctx, axis, index, 3);
 }
 
 int sfd_alloc(sfd_ctx* ctx, size_t bytes, void** out)

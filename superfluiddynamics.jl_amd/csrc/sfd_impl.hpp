@@ -234,21 +234,22 @@ int launch_sweep(sfd_ctx* c, int axis, SweepArgs a)
                             !(a.flags & kFlagScale) && !((a.flags & kFlagNL) && (a.V || a.save)) &&
                             a.phase_row_stride == 0;
     const bool blu = c->blu[axis];
-    Launch L = sweep_geometry(c, axis, a, persistent && !blu);
-    a.tw = c->tw[axis];
+    constexpr bool twice = rt_twice(MODE);  // the double round trips' own variant
+    const int var = twice ? c->var2[axis] : c->var[axis];
+    Launch L = sweep_geometry(c, axis, a, persistent && !blu, twice);
+    a.tw = twice ? c->tw2[axis] : c->tw[axis];
     a.nlen = c->n[axis];
     if (c->time_sweeps)
         SFD_TRY(timer_begin(c, rt_twice(MODE) ? (axis == 0 ? 5 : 3) : (is_round_trip(MODE) ? axis : 4)));
     bool ok = false;
     if constexpr (rt_energy(MODE)) {
         if (blu) return fail(SFD_ERR_SHAPE, "no energy sums fused into Bluestein sweeps (n = %d)", c->n[axis]);
-        ok = launch_variant<P, MODE>(c->n[axis], axis == 0, c->var[axis], L.grid, L.block, c->stream, a, persistent);
+        ok = launch_variant<P, MODE>(c->n[axis], axis == 0, var, L.grid, L.block, c->stream, a, persistent);
     } else {
         ok = blu ? launch_blu<P, MODE>(c->m[axis], axis == 0, L.grid, L.block, c->stream, a)
-                 : launch_variant<P, MODE>(c->n[axis], axis == 0, c->var[axis], L.grid, L.block, c->stream, a,
-                                           persistent);
+                 : launch_variant<P, MODE>(c->n[axis], axis == 0, var, L.grid, L.block, c->stream, a, persistent);
     }
-    if (!ok) return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", c->var[axis], c->n[axis]);
+    if (!ok) return fail(SFD_ERR_SHAPE, "no sweep variant %d for line length %d", var, c->n[axis]);
     SFD_HIP(hipGetLastError());
     if (c->time_sweeps) SFD_TRY(timer_end(c));
     c->last_nblocks = L.nblocks;
@@ -263,8 +264,10 @@ int axis_variants(const sfd_ctx* c, int axis)
     return c->blu[axis] ? 1 : n_variants<P>(c->n[axis], axis == 0);
 }
 
+// `which`: 1 the axis's single round trips and other sweeps, 2 its double
+// round trips ([d N d], [x x]), 3 both.  Each keeps its own twiddle table.
 template <typename P>
-int set_variant(sfd_ctx* c, int axis, int idx)
+int set_variant(sfd_ctx* c, int axis, int idx, int which)
 {
     using R = typename P::R;
     const int n = c->m[axis];  // the transform length (n itself unless Bluestein)
@@ -272,25 +275,28 @@ int set_variant(sfd_ctx* c, int axis, int idx)
     if ((c->blu[axis] && idx != 0) || !variant_info<P>(n, axis == 0, idx, &vi))
         return fail(SFD_ERR_ARG, "axis %d: variant %d does not exist (%d variants)", axis, idx,
                     axis_variants<P>(c, axis));
-    c->pipe_occ[axis] = (vi.pipe && !c->blu[axis]) ? variant_occupancy<P>(n, axis == 0, idx) : 0;
+    if (which < 1 || which > 3) return fail(SFD_ERR_ARG, "set_variant: which = %d", which);
+    if (which & 1) c->pipe_occ[axis] = (vi.pipe && !c->blu[axis]) ? variant_occupancy<P>(n, axis == 0, idx) : 0;
     if (c->blu[axis]) vi.pipe = 0;
-    if (c->tw[axis] && c->vi[axis].n1 == vi.n1 && c->vi[axis].n2 == vi.n2 && c->vi[axis].h == vi.h) {
-        c->var[axis] = idx;
-        c->vi[axis] = vi;
-        return SFD_OK;
+    for (int k = 0; k < 2; ++k) {
+        if (!(which & (1 << k))) continue;
+        void*& tw = k ? c->tw2[axis] : c->tw[axis];
+        VariantInfo& cur = k ? c->vi2[axis] : c->vi[axis];
+        if (!(tw && cur.n1 == vi.n1 && cur.n2 == vi.n2 && cur.h == vi.h)) {
+            std::vector<cx<R>> w;
+            fill_twiddles<R>(w, n, vi.n1, vi.n2, vi.h, !c->blu[axis]);
+            if (c->blu[axis]) fill_bluestein<R>(w, c->n[axis], n);
+            if (tw) {
+                SFD_HIP(hipStreamSynchronize(c->stream));
+                SFD_HIP(hipFree(tw));
+                tw = nullptr;
+            }
+            SFD_HIP(hipMalloc(&tw, w.size() * sizeof(cx<R>)));
+            SFD_TRY(upload_sync(tw, w.data(), w.size() * sizeof(cx<R>)));
+        }
+        (k ? c->var2[axis] : c->var[axis]) = idx;
+        cur = vi;
     }
-    std::vector<cx<R>> w;
-    fill_twiddles<R>(w, n, vi.n1, vi.n2, vi.h, !c->blu[axis]);
-    if (c->blu[axis]) fill_bluestein<R>(w, c->n[axis], n);
-    if (c->tw[axis]) {
-        SFD_HIP(hipStreamSynchronize(c->stream));
-        SFD_HIP(hipFree(c->tw[axis]));
-        c->tw[axis] = nullptr;
-    }
-    SFD_HIP(hipMalloc(&c->tw[axis], w.size() * sizeof(cx<R>)));
-    SFD_TRY(upload_sync(c->tw[axis], w.data(), w.size() * sizeof(cx<R>)));
-    c->var[axis] = idx;
-    c->vi[axis] = vi;
     return SFD_OK;
 }
 
@@ -885,19 +891,19 @@ int autotune_impl(sfd_ctx* c, void* scratch, float* times_out)
     }
     for (int ax = 0; rc == SFD_OK && ax < c->ndim; ++ax) {
         const int nv = axis_variants<P>(c, ax);
-        int best = 0;
-        float best_ms = 1e30f;
+        int best[2] = {0, 0};
+        float best_ms[2] = {1e30f, 1e30f};
         // Chained real-time steps run single round trips, x's with the
         // nonlinear phase in its epilogue.  The innermost axis of a
         // one-step call, of imaginary-time and of energy-loop steps also runs
         // the double round trip [d N d] (z in 3D; y in 2D without rotation),
         // and the x axis of their chained steps the double round trip [x x]:
-        // their variants are chosen by the sum of both times.
+        // the single and the double round trips each get their fastest
+        // variant.
         const bool folds = c->ndim >= 2 && (ax == c->ndim - 1 || (ax == 0 && !c->blu[0]));
         const int single_flags = (c->ndim >= 2 && ax == 0) ? kFlagNL : 0;
         for (int v = 0; rc == SFD_OK && v < nv; ++v) {
-            if ((rc = set_variant<P>(c, ax, v)) != SFD_OK) break;
-            float score = 0.f;
+            if ((rc = set_variant<P>(c, ax, v, 3)) != SFD_OK) break;
             for (int twice = 0; rc == SFD_OK && twice <= (folds ? 1 : 0); ++twice) {
                 auto launch = [&]() {
                     return twice ? round_trip<P>(c, buf, nullptr, ax, 0.0, -0.5, 0.0, 0, ax == 0 ? 0 : kFlagNL, 0.0,
@@ -919,11 +925,14 @@ int autotune_impl(sfd_ctx* c, void* scratch, float* times_out)
                     if (ms / 8 < vbest) vbest = ms / 8;
                 }
                 if (!twice && times_out) times_out[ax * 8 + (v < 8 ? v : 7)] = vbest;
-                score += vbest;
+                if (vbest < best_ms[twice]) {
+                    best_ms[twice] = vbest;
+                    best[twice] = v;
+                }
             }
-            if (score < best_ms) { best_ms = score; best = v; }
         }
-        if (rc == SFD_OK) rc = set_variant<P>(c, ax, best);
+        if (rc == SFD_OK) rc = set_variant<P>(c, ax, best[0], 1);
+        if (rc == SFD_OK) rc = set_variant<P>(c, ax, folds ? best[1] : best[0], 2);
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -940,7 +949,7 @@ int init_tables(sfd_ctx* c)
 {
     for (int ax = 0; ax < c->ndim; ++ax) {
         const int n = c->n[ax];
-        SFD_TRY(set_variant<P>(c, ax, 0));
+        SFD_TRY(set_variant<P>(c, ax, 0, 3));
         std::vector<double> xi(n), co(n);
         for (int i = 0; i < n; ++i) {
             xi[i] = fft_freq(n, c->d[ax], i);

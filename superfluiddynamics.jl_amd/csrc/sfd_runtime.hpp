@@ -184,6 +184,11 @@ struct sfd_ctx {
     int var[3] = {0, 0, 0};
     sfd::VariantInfo vi[3] = {};
     void* tw[3] = {nullptr, nullptr, nullptr};
+    // the variant, shape and twiddles of the axis's double round trips
+    // ([d N d], [x x]), chosen apart from its single round trips
+    int var2[3] = {0, 0, 0};
+    sfd::VariantInfo vi2[3] = {};
+    void* tw2[3] = {nullptr, nullptr, nullptr};
     int tuned = 0;
     int pipe_occ[3] = {0, 0, 0};  // resident workgroups per CU (persistent variants)
     int ncu = 1;
@@ -373,7 +378,7 @@ int upload_sync(void* dst, const void* src, size_t bytes);
 int phase_rows(const sfd_ctx* c, int axis, double Om);
 int ensure_buf(sfd_ctx* c, double** buf, long long* cap, long long need);
 int ensure_partial(sfd_ctx* c, long long need);
-Launch sweep_geometry(const sfd_ctx* c, int axis, SweepArgs& a, bool persistent);
+Launch sweep_geometry(const sfd_ctx* c, int axis, SweepArgs& a, bool persistent, bool twice = false);
 long long sweep_nblocks(const sfd_ctx* c, int axis);
 int check_ctx(sfd_ctx* c);
 long long grid_blocks(long long n);
@@ -404,7 +409,7 @@ struct PrecOps {
                    double Om, double dt, int64_t N, int64_t every, double* gradV, double* grad_beta, double* loss,
                    void* psi_final);
     int (*autotune)(sfd_ctx* c, void* scratch, float* times_out);
-    int (*set_variant)(sfd_ctx* c, int axis, int idx);
+    int (*set_variant)(sfd_ctx* c, int axis, int idx, int which);  // which: 1 single, 2 double, 3 both
     int (*n_variants)(const sfd_ctx* c, int axis);
     int (*build_phase_into)(const sfd_ctx* c, hipStream_t stream, void* dev, int axis, double tau, double cD,
                             double Om, int imag, int rows);
