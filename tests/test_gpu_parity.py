@@ -242,3 +242,15 @@ def test_autotune_keeps_results(gpu, orc):
     p = gpu.GrossPitaevskiiParameters(pot=gpu.PotentialZero(f), coeffΔ=-0.5, β=2.0)
     gpu.NumModelSplit2(f, p, 2e-3, 3, 3).steps(3)
     assert rel_l2(f.to_numpy(), ref) < F32_TOL
+    # the double round trips ([z N z] of one-step calls, [x x] of chained
+    # imaginary-time steps) run the variants the tuner chose for them, which
+    # may differ from the single round trips'
+    m = gpu.NumModelSplit2(f, p, 2e-3, 1, 1)
+    f.set(psi)
+    for _ in range(3):
+        m.steps(1)
+    assert rel_l2(f.to_numpy(), ref) < F32_TOL
+    ref_imag = orc.split_step(psi.copy(), bounds, imag=True, **kw)
+    f.set(psi)
+    gpu.NumModelSplitImag(f, p, 2e-3, 3, 3).steps(3)
+    assert rel_l2(f.to_numpy(), ref_imag) < F32_TOL
