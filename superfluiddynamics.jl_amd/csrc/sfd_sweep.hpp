@@ -526,7 +526,8 @@ __device__ __forceinline__ void radix2_inv(cx<R> (&w)[N1 / N2][N2], const cx<R>*
 // EN adds the energy sums of the state entering the sweep (after a pending
 // scale) to eacc[4], in FP64: [0] sum xi^2 |psi_hat|^2 over the transformed
 // line, and with kFlagPointwise in a contiguous (x) sweep [1] sum |psi|^2,
-// [2] sum V|psi|^2, [3] sum |psi|^4 (vp must then hold V).
+// [2] sum V|psi|^2, [3] sum |psi|^4 (vp must then hold V).  FP32 arithmetic
+// sums a lane's N1 points in FP32 first (one FP64 add per sum and lane).
 // H = 2: the line is two interleaved halves joined by the radix-2 stage; g is
 // then the thread's half-line group in LDS, t its sub-transform lane and h
 // its half; twC the radix-2 twiddles (unused for H = 1).
@@ -557,13 +558,30 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
     }
     if constexpr (EN && TS == 1) {  // the x sweep (contiguous flavour) takes the pointwise sums
         if (flags & kFlagPointwise) {
+            if constexpr (sizeof(R) == 4) {
+                // FP32 sums over the lane's N1 sites, then one FP64 add each:
+                // the double round trips that carry these sums are bound by
+                // VALU issue, where FP64 arithmetic runs at half rate
+                float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
 #pragma unroll
-            for (int j = 0; j < N1; ++j) {
-                const double re = v[j].re, im = v[j].im;
-                const double a2 = re * re + im * im;
-                eacc[1] += a2;
-                eacc[2] += (double)vp[j] * a2;
-                eacc[3] += a2 * a2;
+                for (int j = 0; j < N1; ++j) {
+                    const float a2 = fmaf(v[j].re, v[j].re, v[j].im * v[j].im);
+                    s1 += a2;
+                    s2 = fmaf(vp[j], a2, s2);
+                    s3 = fmaf(a2, a2, s3);
+                }
+                eacc[1] += s1;
+                eacc[2] += s2;
+                eacc[3] += s3;
+            } else {
+#pragma unroll
+                for (int j = 0; j < N1; ++j) {
+                    const double re = v[j].re, im = v[j].im;
+                    const double a2 = re * re + im * im;
+                    eacc[1] += a2;
+                    eacc[2] += (double)vp[j] * a2;
+                    eacc[3] += a2 * a2;
+                }
             }
         }
     }
@@ -580,16 +598,33 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
         // table's own product)
         constexpr int N = H * M;
         const double dk = a.xi[1];
+        if constexpr (sizeof(R) == 4) {
+            // sum kk^2 |psi_hat|^2 over the lane's points in FP32 (kk^2 is
+            // exact), then one FP64 add of dk^2 times it
+            float s0 = 0.0f;
 #pragma unroll
-        for (int r = 0; r < RB; ++r)
+            for (int r = 0; r < RB; ++r)
 #pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) {
-                const int d = RB == 1 ? dct : (r >= RB / 2 ? 1 : 0);
-                const int k = CT ? t + N2 * r + N1 * ((k2 - d) & (N2 - 1)) : t + M * h + N2 * r + N1 * k2;
-                const double x = (double)(k < N / 2 ? k : k - N) * dk;
-                const double re = w[r][k2].re, im = w[r][k2].im;
-                eacc[0] += x * x * (re * re + im * im);
-            }
+                for (int k2 = 0; k2 < N2; ++k2) {
+                    const int d = RB == 1 ? dct : (r >= RB / 2 ? 1 : 0);
+                    const int k = CT ? t + N2 * r + N1 * ((k2 - d) & (N2 - 1)) : t + M * h + N2 * r + N1 * k2;
+                    const float kf = (float)(k < N / 2 ? k : k - N);
+                    const float p = fmaf(w[r][k2].re, w[r][k2].re, w[r][k2].im * w[r][k2].im);
+                    s0 = fmaf(kf * kf, p, s0);
+                }
+            eacc[0] += (double)s0 * (dk * dk);
+        } else {
+#pragma unroll
+            for (int r = 0; r < RB; ++r)
+#pragma unroll
+                for (int k2 = 0; k2 < N2; ++k2) {
+                    const int d = RB == 1 ? dct : (r >= RB / 2 ? 1 : 0);
+                    const int k = CT ? t + N2 * r + N1 * ((k2 - d) & (N2 - 1)) : t + M * h + N2 * r + N1 * k2;
+                    const double x = (double)(k < N / 2 ? k : k - N) * dk;
+                    const double re = w[r][k2].re, im = w[r][k2].im;
+                    eacc[0] += x * x * (re * re + im * im);
+                }
+        }
     }
     // the phase entry of w[r][k2] at that index; for RB = 1 the k2 = 0 entry
     // of lanes with d = 1 wraps to the end of the row
