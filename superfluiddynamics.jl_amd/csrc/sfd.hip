@@ -490,9 +490,9 @@ int batch_solve(sfd_batch* b, void* const* psi, const void* const* V, const doub
     const bool use_graph = (flags & SFD_BATCH_GRAPH) != 0;
     for (int i = 0; i < ni; ++i)
         if (!psi[i]) return fail(SFD_ERR_NULL, "batch: field %d is null", i);
-    if (escalate && !b->lv[1].prec) {
-        b->lv[1].prec = SFD_F64;
+    if (escalate && !b->lv[1].prec) {  // marked set up only once it is
         SFD_TRY(ops_for(SFD_F64).batch_level_setup(b, 1));
+        b->lv[1].prec = SFD_F64;
     }
     batch_kinds(b, Omega);
     batch_drop_graphs(b);
@@ -896,8 +896,8 @@ int sfd_batch_autotune(sfd_batch* b, float* times_ms)
     if (b->prec == SFD_F64) return SFD_OK;
     // the FP64 level that escalated instances finish on has its own variants
     if (!b->lv[1].prec) {
-        b->lv[1].prec = SFD_F64;
         SFD_TRY(ops_for(SFD_F64).batch_level_setup(b, 1));
+        b->lv[1].prec = SFD_F64;
     }
     return ops_for(SFD_F64).batch_autotune(b, 1, nullptr);
 }
