@@ -689,13 +689,20 @@ __device__ __forceinline__ void round_trip_tile(cx<R> (&v)[N1], const R (&vp)[N1
         }
     }
     if (flags & kFlagNorm) {
-        double s = 0.0;
+        if constexpr (sizeof(R) == 4) {  // the lane's N1 points in FP32, then one FP64 add
+            float s = 0.0f;
 #pragma unroll
-        for (int j = 0; j < N1; ++j) {
-            const double re = v[j].re, im = v[j].im;
-            s += re * re + im * im;
+            for (int j = 0; j < N1; ++j) s = fmaf(v[j].re, v[j].re, fmaf(v[j].im, v[j].im, s));
+            if (gm.active) norm_acc += (double)s;
+        } else {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < N1; ++j) {
+                const double re = v[j].re, im = v[j].im;
+                s += re * re + im * im;
+            }
+            if (gm.active) norm_acc += s;
         }
-        if (gm.active) norm_acc += s;
     }
 }
 
@@ -1298,13 +1305,20 @@ __device__ __forceinline__ void round_trip_blu(cx<R> (&v)[N1], const R (&vp)[N1]
         }
     }
     if (flags & kFlagNorm) {
-        double s = 0.0;
+        if constexpr (sizeof(R) == 4) {  // the lane's N1 points in FP32, then one FP64 add
+            float s = 0.0f;
 #pragma unroll
-        for (int j = 0; j < N1; ++j) {
-            const double re = v[j].re, im = v[j].im;
-            s += re * re + im * im;
+            for (int j = 0; j < N1; ++j) s = fmaf(v[j].re, v[j].re, fmaf(v[j].im, v[j].im, s));
+            if (gm.active) norm_acc += (double)s;
+        } else {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < N1; ++j) {
+                const double re = v[j].re, im = v[j].im;
+                s += re * re + im * im;
+            }
+            if (gm.active) norm_acc += s;
         }
-        if (gm.active) norm_acc += s;
     }
 }
 
