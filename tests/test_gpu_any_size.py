@@ -30,8 +30,8 @@ CASES = [
     ((2048, 8), ((-10, 10), (-1, 1)), 0.0, True),
     ((8, 2048), ((-1, 1), (-10, 10)), 0.2, False),
     ((24, 2048), ((-2, 2), (-10, 10)), 0.0, True),
-    # power-of-two x with Bluestein y / z at Omega = 0: chained steps ([x x])
-    # around Bluestein sweeps
+    # power-of-two x with Bluestein y / z at Omega = 0: chained steps around
+    # Bluestein sweeps (real time: merged flows; imaginary time: [x x])
     ((64, 48), ((-6, 6), (-5, 5)), 0.0, True),
     ((32, 24, 20), ((-4, 4), (-3, 3), (-3, 3)), 0.0, True),
 ]
